@@ -1,0 +1,133 @@
+// Density-matrix kernels for gfx950: diagonal reductions, collapse and the
+// one-qubit partial trace.  rho is row-major dim x dim c128 (dim = 2^n), i.e.
+// a 2n-qubit vector whose element (i, j) sits at (i << n) | j.  Gate and
+// channel application reuse the statevector kernels on that vector (a gate
+// with its channel is one 4x4 superoperator on bits (n+t, t)).
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "dm_kernels.hpp"
+
+namespace ev::kern {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+#define GRID_STRIDE(i, count)                                                               \
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (count);         \
+         i += uint64_t(gridDim.x) * blockDim.x)
+
+inline dim3 rgrid(uint64_t work)
+{
+    uint64_t b = (work + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > uint64_t(kReduceBlocks)) b = kReduceBlocks;
+    return dim3(static_cast<unsigned>(b));
+}
+
+inline dim3 grid_for(uint64_t work)
+{
+    uint64_t b = (work + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > (1u << 16)) b = 1u << 16;
+    return dim3(static_cast<unsigned>(b));
+}
+
+template <int NV> __device__ void block_sum(double (&v)[NV], double *partials)
+{
+    __shared__ double sh[NV][kBlock];
+    for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < unsigned(s))
+            for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) partials[NV * blockIdx.x + c] = sh[c][0];
+}
+
+template <int NV> __global__ void k_finalize_dm(const double *partials, int nblocks, double *out)
+{
+    __shared__ double sh[NV][1024];
+    for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] = threadIdx.x < unsigned(nblocks) ? partials[NV * threadIdx.x + c] : 0.0;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < unsigned(s))
+            for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) out[c] = sh[c][0];
+}
+
+// density.rs:121-124 (mask = 0) and density_matrix.rs:175-186 (mask = bit q):
+// sum of Re rho[j][j] over j with (j & mask) == mask.
+__global__ void k_diag_partial(const double2 *rho, int n, uint64_t mask, double *partials)
+{
+    double v[1] = {0.0};
+    const uint64_t dim = uint64_t(1) << n;
+    GRID_STRIDE(j, dim)
+    {
+        if ((j & mask) == mask) v[0] += rho[(j << n) | j].x;
+    }
+    block_sum<1>(v, partials);
+}
+
+// density_matrix.rs:189-207: zero every entry whose row or column bit q differs
+// from the outcome.
+__global__ void k_dm_collapse(double2 *rho, int n, int q, int outcome)
+{
+    const uint64_t len = uint64_t(1) << (2 * n);
+    GRID_STRIDE(x, len)
+    {
+        const bool ok = (((x >> (n + q)) & 1) == uint64_t(outcome)) && (((x >> q) & 1) == uint64_t(outcome));
+        if (!ok) rho[x] = make_double2(0.0, 0.0);
+    }
+}
+
+// density.rs:270-288: reduced[iq*2+jq] = sum over the other bits of rho[i][j].
+__global__ void k_ptrace_partial(const double2 *rho, int n, int q, double *partials)
+{
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t half = uint64_t(1) << (n - 1), bit = uint64_t(1) << q;
+    GRID_STRIDE(x, half)
+    {
+        const uint64_t b = insert0(x, q);
+#pragma unroll
+        for (int iq = 0; iq < 2; ++iq)
+#pragma unroll
+            for (int jq = 0; jq < 2; ++jq) {
+                const uint64_t i = b | (iq ? bit : 0), j = b | (jq ? bit : 0);
+                const double2 e = rho[(i << n) | j];
+                v[2 * (iq * 2 + jq)] += e.x;
+                v[2 * (iq * 2 + jq) + 1] += e.y;
+            }
+    }
+    block_sum<8>(v, partials);
+}
+
+}  // namespace
+
+void dm_diag_sum(const void *rho, int n, uint64_t mask, double *partials, double *out, hipStream_t st)
+{
+    const dim3 g = rgrid(uint64_t(1) << n);
+    hipLaunchKernelGGL(k_diag_partial, g, dim3(kBlock), 0, st, static_cast<const double2 *>(rho), n, mask, partials);
+    hipLaunchKernelGGL(k_finalize_dm<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void dm_collapse(void *rho, int n, int q, int outcome, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dm_collapse, grid_for(uint64_t(1) << (2 * n)), dim3(kBlock), 0, st, static_cast<double2 *>(rho), n,
+                       q, outcome);
+}
+
+void dm_partial_trace(const void *rho, int n, int q, double *partials, double *out, hipStream_t st)
+{
+    const dim3 g = rgrid(uint64_t(1) << (n - 1));
+    hipLaunchKernelGGL(k_ptrace_partial, g, dim3(kBlock), 0, st, static_cast<const double2 *>(rho), n, q, partials);
+    hipLaunchKernelGGL(k_finalize_dm<8>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+}  // namespace ev::kern

@@ -1,0 +1,534 @@
+// Per-gate statevector kernels for gfx950 (one HBM pass per gate), FP64
+// reductions, collapse, precision conversion and Born-rule sampling.
+//
+// Layout: amplitudes are contiguous {re, im} pairs (double2 for c128, float2
+// for c64), index bit q = qubit q.  Every kernel is a grid-stride loop over a
+// bijective index map (bit insertion, src/kernel.rs:37-66), so each amplitude
+// is read and written by exactly one thread: consecutive threads take
+// consecutive counters, which keeps both halves of every pair coalesced.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "circuit.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace ev::kern {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint64_t kMaxGrid = 1u << 16;
+
+inline dim3 grid_for(uint64_t work)
+{
+    uint64_t b = (work + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > kMaxGrid) b = kMaxGrid;
+    return dim3(static_cast<unsigned>(b));
+}
+
+#define GRID_STRIDE(i, count)                                                               \
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (count);         \
+         i += uint64_t(gridDim.x) * blockDim.x)
+
+template <class R> M2<R> to_m2(const c128 m[4])
+{
+    M2<R> o;
+    for (int i = 0; i < 4; ++i) o.m[i] = mk<Amp<R>>(static_cast<R>(m[i].re), static_cast<R>(m[i].im));
+    return o;
+}
+template <class R> M4<R> to_m4(const c128 m[16])
+{
+    M4<R> o;
+    for (int i = 0; i < 16; ++i) o.m[i] = mk<Amp<R>>(static_cast<R>(m[i].re), static_cast<R>(m[i].im));
+    return o;
+}
+
+// ------------------------------------------------------------------ kernels
+
+template <class R> __global__ void k_init_basis(Amp<R> *psi, uint64_t len, uint64_t basis)
+{
+    GRID_STRIDE(i, len) psi[i] = mk<Amp<R>>(R(i == basis ? 1 : 0), R(0));
+}
+
+// kernel.rs:100-119: new0 = m00 a0 + m01 a1, new1 = m10 a0 + m11 a1.
+template <class R> __global__ void k_apply1_dense(Amp<R> *psi, uint64_t npairs, int k, M2<R> g)
+{
+    const uint64_t bit = uint64_t(1) << k;
+    GRID_STRIDE(i, npairs)
+    {
+        const uint64_t i0 = insert0(i, k), i1 = i0 | bit;
+        const Amp<R> a0 = psi[i0], a1 = psi[i1];
+        psi[i0] = cadd(cmul(g.m[0], a0), cmul(g.m[1], a1));
+        psi[i1] = cadd(cmul(g.m[2], a0), cmul(g.m[3], a1));
+    }
+}
+
+// Diagonal gate: with m01 = m10 = 0 the reference's sums reduce exactly to
+// m00*a0 and m11*a1 (the zero products contribute signed zeros only).
+template <class R> __global__ void k_apply1_diag(Amp<R> *psi, uint64_t npairs, int k, Amp<R> d0, Amp<R> d1)
+{
+    const uint64_t bit = uint64_t(1) << k;
+    GRID_STRIDE(i, npairs)
+    {
+        const uint64_t i0 = insert0(i, k), i1 = i0 | bit;
+        psi[i0] = cmul(d0, psi[i0]);
+        psi[i1] = cmul(d1, psi[i1]);
+    }
+}
+
+// Phase-type diagonal (m00 == 1 exactly): 1*a0 is a0 bit for bit, so only the
+// bit-k-set half moves through HBM.
+template <class R> __global__ void k_apply1_phase(Amp<R> *psi, uint64_t npairs, int k, Amp<R> d1)
+{
+    const uint64_t bit = uint64_t(1) << k;
+    GRID_STRIDE(i, npairs)
+    {
+        const uint64_t i1 = insert0(i, k) | bit;
+        psi[i1] = cmul(d1, psi[i1]);
+    }
+}
+
+__device__ __forceinline__ uint64_t quad_base(uint64_t i, int lo, int hi) { return insert0(insert0(i, lo), hi); }
+
+// kernel.rs:248-260: swap i10 <-> i11 (a = control, b = target).
+template <class R> __global__ void k_cnot(Amp<R> *psi, uint64_t ngroups, int lo, int hi, uint64_t ba, uint64_t bb)
+{
+    GRID_STRIDE(i, ngroups)
+    {
+        const uint64_t base = quad_base(i, lo, hi);
+        const Amp<R> t = psi[base | ba];
+        psi[base | ba] = psi[base | ba | bb];
+        psi[base | ba | bb] = t;
+    }
+}
+
+// kernel.rs:277-289: swap i01 <-> i10.
+template <class R> __global__ void k_swap(Amp<R> *psi, uint64_t ngroups, int lo, int hi, uint64_t ba, uint64_t bb)
+{
+    GRID_STRIDE(i, ngroups)
+    {
+        const uint64_t base = quad_base(i, lo, hi);
+        const Amp<R> t = psi[base | bb];
+        psi[base | bb] = psi[base | ba];
+        psi[base | ba] = t;
+    }
+}
+
+// kernel.rs:263-274: negate i11.
+template <class R> __global__ void k_cz(Amp<R> *psi, uint64_t ngroups, int lo, int hi, uint64_t both)
+{
+    GRID_STRIDE(i, ngroups)
+    {
+        const uint64_t j = quad_base(i, lo, hi) | both;
+        const Amp<R> v = psi[j];
+        psi[j] = mk<Amp<R>>(-v.x, -v.y);
+    }
+}
+
+// kernel.rs:214-243: dense 4x4 on [i00, i01, i10, i11], sums left to right.
+template <class R>
+__global__ void k_apply2_dense(Amp<R> *psi, uint64_t ngroups, int lo, int hi, uint64_t ba, uint64_t bb, M4<R> g)
+{
+    GRID_STRIDE(i, ngroups)
+    {
+        const uint64_t base = quad_base(i, lo, hi);
+        const uint64_t idx[4] = {base, base | bb, base | ba, base | ba | bb};
+        const Amp<R> a[4] = {psi[idx[0]], psi[idx[1]], psi[idx[2]], psi[idx[3]]};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const Amp<R> *row = g.m + 4 * r;
+            psi[idx[r]] =
+                cadd(cadd(cadd(cmul(row[0], a[0]), cmul(row[1], a[1])), cmul(row[2], a[2])), cmul(row[3], a[3]));
+        }
+    }
+}
+
+struct CtrlArgs {
+    int npos;            // controls + target, ascending
+    int pos[EV_MAX_CONTROLS + 1];
+    uint64_t ctrl_mask;  // all control bits
+    uint64_t tbit;
+};
+
+// kernel.rs:332-358, enumerating only the pairs whose controls are all set:
+// insert zeros at every control and target position, then set the controls.
+template <class R> __global__ void k_ctrl1(Amp<R> *psi, uint64_t nwork, CtrlArgs a, M2<R> g)
+{
+    GRID_STRIDE(i, nwork)
+    {
+        uint64_t x = i;
+        for (int p = 0; p < a.npos; ++p) x = insert0(x, a.pos[p]);
+        const uint64_t i0 = x | a.ctrl_mask, i1 = i0 | a.tbit;
+        const Amp<R> a0 = psi[i0], a1 = psi[i1];
+        psi[i0] = cadd(cmul(g.m[0], a0), cmul(g.m[1], a1));
+        psi[i1] = cadd(cmul(g.m[2], a0), cmul(g.m[3], a1));
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+
+__device__ __forceinline__ double nsq(double2 a) { return __builtin_fma(a.x, a.x, a.y * a.y); }  // complex.rs:80-82
+__device__ __forceinline__ double nsq(float2 a)
+{
+    const double re = a.x, im = a.y;
+    return __builtin_fma(re, re, im * im);
+}
+__device__ __forceinline__ double2 widen(double2 a) { return a; }
+__device__ __forceinline__ double2 widen(float2 a) { return make_double2(a.x, a.y); }
+
+// Block tree-sum of NV doubles per thread into LDS, thread 0 writes partial.
+template <int NV> __device__ void block_sum(double (&v)[NV], double *partials)
+{
+    __shared__ double sh[NV][kBlock];
+    for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) partials[NV * blockIdx.x + c] = sh[c][0];
+}
+
+template <class R> __global__ void k_norm_partial(const Amp<R> *psi, uint64_t len, double *partials)
+{
+    double v[1] = {0.0};
+    GRID_STRIDE(i, len) v[0] += nsq(psi[i]);
+    block_sum<1>(v, partials);
+}
+
+template <class R> __global__ void k_p1_partial(const Amp<R> *psi, uint64_t half, int k, double *partials)
+{
+    double v[1] = {0.0};
+    const uint64_t bit = uint64_t(1) << k;
+    GRID_STRIDE(i, half) v[0] += nsq(psi[insert0(i, k) | bit]);
+    block_sum<1>(v, partials);
+}
+
+// state.rs:130-138: sum_j conj(other_j) * self_j in f64.
+template <class R> __global__ void k_overlap_partial(const Amp<R> *s, const Amp<R> *o, uint64_t len, double *partials)
+{
+    double v[2] = {0.0, 0.0};
+    GRID_STRIDE(i, len)
+    {
+        const double2 a = widen(s[i]), b = widen(o[i]);
+        const double2 p = cmul(make_double2(b.x, -b.y), a);
+        v[0] += p.x;
+        v[1] += p.y;
+    }
+    block_sum<2>(v, partials);
+}
+
+// state.rs:163-185: r00 = sum |a_j|^2 (bit clear), r11 (bit set), r01 = sum a_j conj(a_j|bit).
+template <class R> __global__ void k_bloch_partial(const Amp<R> *psi, uint64_t half, int k, double *partials)
+{
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint64_t bit = uint64_t(1) << k;
+    GRID_STRIDE(i, half)
+    {
+        const uint64_t j0 = insert0(i, k);
+        const double2 a = widen(psi[j0]), b = widen(psi[j0 | bit]);
+        v[0] += nsq(a);
+        v[1] += nsq(b);
+        const double2 p = cmul(a, make_double2(b.x, -b.y));
+        v[2] += p.x;
+        v[3] += p.y;
+    }
+    block_sum<4>(v, partials);
+}
+
+template <int NV> __global__ void k_finalize(const double *partials, int nblocks, double *out)
+{
+    __shared__ double sh[NV][1024];
+    for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] = threadIdx.x < unsigned(nblocks) ? partials[NV * threadIdx.x + c] : 0.0;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < unsigned(s))
+            for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) out[c] = sh[c][0];
+}
+
+template <class R> __global__ void k_collapse(Amp<R> *psi, uint64_t len, int k, int outcome, double scale)
+{
+    GRID_STRIDE(j, len)
+    {
+        const bool keep = (((j >> k) & 1) != 0) == (outcome != 0);
+        const Amp<R> a = psi[j];
+        psi[j] = keep ? mk<Amp<R>>(R(a.x * scale), R(a.y * scale)) : mk<Amp<R>>(R(0), R(0));
+    }
+}
+
+template <class R> __global__ void k_scale(Amp<R> *psi, uint64_t len, double s)
+{
+    GRID_STRIDE(j, len)
+    {
+        const Amp<R> a = psi[j];
+        psi[j] = mk<Amp<R>>(R(a.x * s), R(a.y * s));
+    }
+}
+
+__global__ void k_c128_to_c64(const double2 *src, float2 *dst, uint64_t len)
+{
+    GRID_STRIDE(j, len) dst[j] = make_float2(float(src[j].x), float(src[j].y));
+}
+__global__ void k_c64_to_c128(const float2 *src, double2 *dst, uint64_t len)
+{
+    GRID_STRIDE(j, len) dst[j] = make_double2(src[j].x, src[j].y);
+}
+
+// ----------------------------------------------------------------- sampling
+// The f64 CDF is defined chunk-wise: chunk c covers indices [c*C, (c+1)*C);
+// its total is the sequential sum of |amp|^2 over the chunk; prefix[c] is the
+// sequential sum of totals up to and including c.  A shot with uniform u
+// targets u * prefix[last] and returns the first index whose running sum
+// (prefix[c-1] + sequential in-chunk sum) exceeds the target.
+
+constexpr int kChunkLog = 10;
+
+template <class R> __global__ void k_chunk_totals(const Amp<R> *psi, uint64_t nchunks, uint64_t csize, double *tot)
+{
+    GRID_STRIDE(c, nchunks)
+    {
+        double s = 0.0;
+        const Amp<R> *p = psi + c * csize;
+        for (uint64_t j = 0; j < csize; ++j) s += nsq(p[j]);
+        tot[c] = s;
+    }
+}
+
+// Sequential inclusive prefix over the chunk totals (one thread; at most
+// 2^24 chunks), so the CDF has one fixed summation order.
+__global__ void k_prefix(double *tot, uint64_t nchunks)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double run = 0.0;
+    for (uint64_t c = 0; c < nchunks; ++c) {
+        run += tot[c];
+        tot[c] = run;
+    }
+}
+
+template <class R>
+__global__ void k_sample(const Amp<R> *psi, uint64_t nchunks, uint64_t csize, const double *prefix, const double *u,
+                         uint64_t shots, uint64_t *out)
+{
+    GRID_STRIDE(s, shots)
+    {
+        const double target = u[s] * prefix[nchunks - 1];
+        uint64_t lo = 0, hi = nchunks - 1;  // first c with prefix[c] > target (last chunk if none)
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (prefix[mid] > target) hi = mid; else lo = mid + 1;
+        }
+        double run = lo > 0 ? prefix[lo - 1] : 0.0;
+        uint64_t pick = lo * csize + csize - 1, last_nz = ~uint64_t(0);
+        bool found = false;
+        for (uint64_t j = 0; j < csize; ++j) {
+            const double w = nsq(psi[lo * csize + j]);
+            if (w > 0.0) last_nz = lo * csize + j;
+            run += w;
+            if (run > target) {
+                pick = lo * csize + j;
+                found = true;
+                break;
+            }
+        }
+        if (!found && last_nz != ~uint64_t(0)) pick = last_nz;
+        out[s] = pick;
+    }
+}
+
+template <class R> Amp<R> *A(void *p) { return static_cast<Amp<R> *>(p); }
+template <class R> const Amp<R> *A(const void *p) { return static_cast<const Amp<R> *>(p); }
+
+template <class R> void apply1_t(void *psi, int n, int k, const c128 m[4], hipStream_t st)
+{
+    const uint64_t npairs = uint64_t(1) << (n - 1);
+    const M2<R> g = to_m2<R>(m);
+    switch (classify1(m)) {
+    case K1::Phase:
+        hipLaunchKernelGGL(k_apply1_phase<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g.m[3]);
+        break;
+    case K1::Diag:
+        hipLaunchKernelGGL(k_apply1_diag<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g.m[0], g.m[3]);
+        break;
+    case K1::Dense:
+        hipLaunchKernelGGL(k_apply1_dense<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g);
+        break;
+    }
+}
+
+template <class R> void apply2_t(void *psi, int n, int a, int b, const c128 m[16], hipStream_t st)
+{
+    const uint64_t ng = uint64_t(1) << (n - 2);
+    const int lo = std::min(a, b), hi = std::max(a, b);
+    const uint64_t ba = uint64_t(1) << a, bb = uint64_t(1) << b;
+    switch (classify2(m)) {
+    case K2::Cnot:
+        hipLaunchKernelGGL(k_cnot<R>, grid_for(ng), dim3(kBlock), 0, st, A<R>(psi), ng, lo, hi, ba, bb);
+        break;
+    case K2::Cz:
+        hipLaunchKernelGGL(k_cz<R>, grid_for(ng), dim3(kBlock), 0, st, A<R>(psi), ng, lo, hi, ba | bb);
+        break;
+    case K2::Swap:
+        hipLaunchKernelGGL(k_swap<R>, grid_for(ng), dim3(kBlock), 0, st, A<R>(psi), ng, lo, hi, ba, bb);
+        break;
+    case K2::Dense:
+        hipLaunchKernelGGL(k_apply2_dense<R>, grid_for(ng), dim3(kBlock), 0, st, A<R>(psi), ng, lo, hi, ba, bb,
+                           to_m4<R>(m));
+        break;
+    }
+}
+
+template <class R>
+void ctrl1_t(void *psi, int n, const uint64_t *controls, int nc, int target, const c128 m[4], hipStream_t st)
+{
+    CtrlArgs a{};
+    a.tbit = uint64_t(1) << target;
+    std::vector<int> pos;
+    for (int c = 0; c < nc; ++c) {
+        a.ctrl_mask |= uint64_t(1) << controls[c];
+        pos.push_back(static_cast<int>(controls[c]));
+    }
+    pos.push_back(target);
+    std::sort(pos.begin(), pos.end());
+    pos.erase(std::unique(pos.begin(), pos.end()), pos.end());  // repeated controls are one condition
+    a.npos = static_cast<int>(pos.size());
+    for (int i = 0; i < a.npos; ++i) a.pos[i] = pos[i];
+    const uint64_t nwork = uint64_t(1) << (n - a.npos);
+    hipLaunchKernelGGL(k_ctrl1<R>, grid_for(nwork), dim3(kBlock), 0, st, A<R>(psi), nwork, a, to_m2<R>(m));
+}
+
+inline dim3 rgrid(uint64_t work)
+{
+    uint64_t b = (work + kBlock - 1) / kBlock;
+    return dim3(static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(b, kReduceBlocks))));
+}
+
+}  // namespace
+
+void init_basis(void *psi, Prec p, int n, uint64_t basis, hipStream_t st)
+{
+    const uint64_t len = uint64_t(1) << n;
+    if (p == C128) hipLaunchKernelGGL(k_init_basis<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, basis);
+    else hipLaunchKernelGGL(k_init_basis<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, basis);
+}
+
+void apply1(void *psi, Prec p, int n, int k, const c128 m[4], hipStream_t st)
+{
+    if (p == C128) apply1_t<double>(psi, n, k, m, st);
+    else apply1_t<float>(psi, n, k, m, st);
+}
+
+void apply2(void *psi, Prec p, int n, int a, int b, const c128 m[16], hipStream_t st)
+{
+    if (p == C128) apply2_t<double>(psi, n, a, b, m, st);
+    else apply2_t<float>(psi, n, a, b, m, st);
+}
+
+void ctrl1(void *psi, Prec p, int n, const uint64_t *controls, int nc, int target, const c128 m[4], hipStream_t st)
+{
+    if (p == C128) ctrl1_t<double>(psi, n, controls, nc, target, m, st);
+    else ctrl1_t<float>(psi, n, controls, nc, target, m, st);
+}
+
+void reduce_norm(const void *psi, Prec p, int n, double *partials, double *out, hipStream_t st)
+{
+    const uint64_t len = uint64_t(1) << n;
+    const dim3 g = rgrid(len);
+    if (p == C128) hipLaunchKernelGGL(k_norm_partial<double>, g, dim3(kBlock), 0, st, A<double>(psi), len, partials);
+    else hipLaunchKernelGGL(k_norm_partial<float>, g, dim3(kBlock), 0, st, A<float>(psi), len, partials);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void reduce_p1(const void *psi, Prec p, int n, int k, double *partials, double *out, hipStream_t st)
+{
+    const uint64_t half = uint64_t(1) << (n - 1);
+    const dim3 g = rgrid(half);
+    if (p == C128) hipLaunchKernelGGL(k_p1_partial<double>, g, dim3(kBlock), 0, st, A<double>(psi), half, k, partials);
+    else hipLaunchKernelGGL(k_p1_partial<float>, g, dim3(kBlock), 0, st, A<float>(psi), half, k, partials);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void reduce_overlap(const void *s, const void *o, Prec p, int n, double *partials, double *out, hipStream_t st)
+{
+    const uint64_t len = uint64_t(1) << n;
+    const dim3 g = rgrid(len);
+    if (p == C128)
+        hipLaunchKernelGGL(k_overlap_partial<double>, g, dim3(kBlock), 0, st, A<double>(s), A<double>(o), len, partials);
+    else hipLaunchKernelGGL(k_overlap_partial<float>, g, dim3(kBlock), 0, st, A<float>(s), A<float>(o), len, partials);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void reduce_bloch(const void *psi, Prec p, int n, int k, double *partials, double *out, hipStream_t st)
+{
+    const uint64_t half = uint64_t(1) << (n - 1);
+    const dim3 g = rgrid(half);
+    if (p == C128) hipLaunchKernelGGL(k_bloch_partial<double>, g, dim3(kBlock), 0, st, A<double>(psi), half, k, partials);
+    else hipLaunchKernelGGL(k_bloch_partial<float>, g, dim3(kBlock), 0, st, A<float>(psi), half, k, partials);
+    hipLaunchKernelGGL(k_finalize<4>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void collapse(void *psi, Prec p, int n, int k, int outcome, double scale, hipStream_t st)
+{
+    const uint64_t len = uint64_t(1) << n;
+    if (p == C128)
+        hipLaunchKernelGGL(k_collapse<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, k, outcome, scale);
+    else hipLaunchKernelGGL(k_collapse<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, k, outcome, scale);
+}
+
+void scale(void *psi, Prec p, int n, double s, hipStream_t st)
+{
+    const uint64_t len = uint64_t(1) << n;
+    if (p == C128) hipLaunchKernelGGL(k_scale<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, s);
+    else hipLaunchKernelGGL(k_scale<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, s);
+}
+
+void c128_to_c64(const void *src, void *dst, uint64_t len, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_c128_to_c64, grid_for(len), dim3(kBlock), 0, st, static_cast<const double2 *>(src),
+                       static_cast<float2 *>(dst), len);
+}
+
+void c64_to_c128(const void *src, void *dst, uint64_t len, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_c64_to_c128, grid_for(len), dim3(kBlock), 0, st, static_cast<const float2 *>(src),
+                       static_cast<double2 *>(dst), len);
+}
+
+size_t sample_scratch_doubles(int n)
+{
+    const int cl = std::min(n, kChunkLog);
+    return size_t(1) << (n - cl);
+}
+
+void sample(const void *psi, Prec p, int n, const double *u, uint64_t shots, double *scratch, uint64_t *out,
+            hipStream_t st)
+{
+    const int cl = std::min(n, kChunkLog);
+    const uint64_t csize = uint64_t(1) << cl, nchunks = uint64_t(1) << (n - cl);
+    if (p == C128)
+        hipLaunchKernelGGL(k_chunk_totals<double>, grid_for(nchunks), dim3(kBlock), 0, st, A<double>(psi), nchunks, csize,
+                           scratch);
+    else
+        hipLaunchKernelGGL(k_chunk_totals<float>, grid_for(nchunks), dim3(kBlock), 0, st, A<float>(psi), nchunks, csize,
+                           scratch);
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(64), 0, st, scratch, nchunks);
+    if (shots == 0) return;
+    if (p == C128)
+        hipLaunchKernelGGL(k_sample<double>, grid_for(shots), dim3(kBlock), 0, st, A<double>(psi), nchunks, csize, scratch,
+                           u, shots, out);
+    else
+        hipLaunchKernelGGL(k_sample<float>, grid_for(shots), dim3(kBlock), 0, st, A<float>(psi), nchunks, csize, scratch, u,
+                           shots, out);
+}
+
+}  // namespace ev::kern

@@ -1,0 +1,259 @@
+/*
+ * everett.h -- C ABI of the MI355X-native everett engine (libeverett.so).
+ *
+ * This is the drop-in boundary for hz2/everett 0.1.2's simulation paths.  Each
+ * entry point names the reference interface it replaces (paths relative to the
+ * crate root).  A Rust crate binds these with `extern "C"` (see INTEGRATION.md
+ * in the repository); the Python package everett_amd binds them with ctypes.
+ *
+ * Conventions
+ *  - Every function returning int returns EV_OK (0) or a negative ev_status.
+ *    The thread's last error text is available from ev_last_error().
+ *  - Opaque handles own their device memory; host arrays are borrowed for the
+ *    duration of the call only.  A handle is not thread-safe (it mirrors the
+ *    reference's `&mut self`); distinct handles may be used from distinct
+ *    threads.
+ *  - The measurement RNG never runs on the device: the host draws uniforms
+ *    from the reference stream (src/rng.rs) in the reference's order.
+ *  - Qubit 0 is the least-significant bit of a basis index (src/qubit.rs:11-13).
+ */
+#ifndef EVERETT_H
+#define EVERETT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* src/complex.rs:22-29: #[repr(C)] Complex64 { re: f64, im: f64 } */
+typedef struct { double re, im; } ev_c128;
+
+/* Storage precision of a state.  EV_C128 is the reference's Complex64 (two
+ * f64).  EV_C64 stores two f32 per amplitude and keeps every norm,
+ * probability and sampling reduction in f64. */
+typedef enum { EV_C128 = 0, EV_C64 = 1 } ev_precision;
+
+typedef enum {
+    EV_OK = 0,
+    EV_E_QUBIT_RANGE = -1, /* Error::QubitOutOfRange        src/error.rs:12-17  a=qubit b=num_qubits */
+    EV_E_CBIT_RANGE = -2,  /* Error::ClassicalBitOutOfRange src/error.rs:19-24  a=bit   b=num_classical */
+    EV_E_DUP_QUBIT = -3,   /* Error::DuplicateQubit         src/error.rs:26-29  a=qubit */
+    EV_E_DIM = -4,         /* Error::DimensionMismatch      src/error.rs:32-37  a=len   b=expected */
+    EV_E_ARG = -5,         /* malformed argument (null pointer, unknown op kind, dangling If prefix, ...) */
+    EV_E_OOM = -6,         /* device memory exhausted */
+    EV_E_HIP = -7,         /* HIP runtime error */
+    EV_E_ADMISSION = -8,   /* serving: the request can never fit the configured memory */
+    EV_E_LIMIT = -9,       /* beyond a documented size limit of this build */
+    EV_E_STATE = -10       /* the handle is not in a state that allows the call */
+} ev_status;
+
+/* Structured error (mirrors src/error.rs:8-53). */
+typedef struct {
+    int32_t code;
+    uint64_t a, b;
+    char message[256];
+} ev_error;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *ev_last_error(void);
+
+/* ------------------------------------------------------------------- ops */
+
+/* src/op.rs:14-58.  Op::IfClassic { bit, then } is encoded as an EV_OP_IF
+ * record (target = classical bit) guarding the record that follows it; nested
+ * IfClassic is a run of EV_OP_IF records. */
+typedef enum {
+    EV_OP_APPLY1 = 1,     /* Op::Apply1 { gate, target }               m[0..4)  */
+    EV_OP_APPLY2 = 2,     /* Op::Apply2 { gate, a=target, b }          m[0..16) */
+    EV_OP_CONTROLLED = 3, /* Op::Controlled { controls, gate, target } m[0..4)  */
+    EV_OP_MEASURE = 4,    /* Op::Measure { qubit=target, into=b } */
+    EV_OP_IF = 5          /* Op::IfClassic { bit=target, then=<next record> } */
+} ev_op_kind;
+
+#define EV_MAX_CONTROLS 16
+
+typedef struct {
+    uint32_t kind;         /* ev_op_kind */
+    uint32_t num_controls; /* EV_OP_CONTROLLED: entries used in controls[] */
+    uint64_t target;
+    uint64_t b;
+    uint64_t controls[EV_MAX_CONTROLS];
+    ev_c128 m[16];         /* row-major gate matrix (src/gate.rs:21-35) */
+} ev_op;
+
+/* --------------------------------------------------------------- circuit */
+
+/* src/circuit.rs:25-30 Circuit { ops, num_qubits, num_classical } */
+typedef struct ev_circuit ev_circuit;
+
+int ev_circuit_create(uint64_t num_qubits, uint64_t num_classical, ev_circuit **out); /* circuit.rs:35-52 */
+void ev_circuit_destroy(ev_circuit *c);
+int ev_circuit_append(ev_circuit *c, const ev_op *ops, size_t count);  /* circuit.rs:91-116, 200-236 */
+int ev_circuit_compose(ev_circuit *dst, const ev_circuit *src);        /* circuit.rs:124-127 */
+size_t ev_circuit_len(const ev_circuit *c);                            /* records, If prefixes included */
+uint64_t ev_circuit_num_qubits(const ev_circuit *c);                   /* circuit.rs:55-58 */
+uint64_t ev_circuit_num_classical(const ev_circuit *c);                /* circuit.rs:62-65 */
+int ev_circuit_export(const ev_circuit *c, ev_op *out, size_t cap, size_t *len);
+int ev_circuit_validate(const ev_circuit *c, ev_error *err);           /* circuit.rs:245-306 */
+
+/* src/rng.rs:9-56 (SplitMix64-seeded xoshiro256++), host only. */
+typedef struct { uint64_t s[4]; } ev_rng;
+void ev_rng_seed(ev_rng *r, uint64_t seed);
+uint64_t ev_rng_next_u64(ev_rng *r);
+double ev_rng_next_f64(ev_rng *r);
+
+/* Circuit generators.  ev_circuit_qft follows src/algorithms/qft.rs:54-62;
+ * the others define the benchmark workloads (docs/configs.md in the
+ * repository gives the exact draw order). */
+int ev_circuit_qft(uint64_t n, ev_circuit **out);
+int ev_circuit_iqft(uint64_t n, ev_circuit **out);                      /* qft.rs:81-91 */
+int ev_circuit_brickwall(uint64_t n, uint64_t depth, ev_rng *rng, ev_circuit **out);
+int ev_circuit_dm_random(uint64_t n, uint64_t depth, ev_rng *rng, ev_circuit **out);
+
+/* ----------------------------------------------------------- statevector */
+
+/* Device-resident statevector: src/state.rs:23-29 State { amps, n }. */
+typedef struct ev_sv ev_sv;
+
+int ev_sv_create(uint64_t n, ev_precision p, ev_sv **out);              /* state.rs:38-44 State::zero */
+int ev_sv_from_amplitudes(const ev_c128 *amps, size_t len, ev_precision p, ev_sv **out); /* state.rs:51-62 */
+void ev_sv_destroy(ev_sv *s);
+uint64_t ev_sv_num_qubits(const ev_sv *s);                              /* state.rs:66-70 */
+ev_precision ev_sv_precision(const ev_sv *s);
+int ev_sv_read(const ev_sv *s, ev_c128 *host, size_t len);              /* state.rs:81-83 amplitudes() (c64 widened) */
+int ev_sv_write(ev_sv *s, const ev_c128 *host, size_t len);
+int ev_sv_copy(const ev_sv *src, ev_sv **out);                          /* State: Clone */
+
+/* Per-gate device kernels (src/kernel.rs:76, :192, :299).  Preconditions are
+ * checked and reported as EV_E_QUBIT_RANGE / EV_E_DUP_QUBIT / EV_E_ARG instead
+ * of the reference's debug_assert!. */
+int ev_sv_apply_1q(ev_sv *s, uint64_t k, const ev_c128 m[4]);
+int ev_sv_apply_2q(ev_sv *s, uint64_t a, uint64_t b, const ev_c128 m[16]);
+int ev_sv_apply_controlled_1q(ev_sv *s, const uint64_t *controls, size_t num_controls, uint64_t target,
+                              const ev_c128 m[4]);
+/* src/measure.rs:15-51 with the uniform `u` drawn by the caller (Rng::next_f64). */
+int ev_sv_measure(ev_sv *s, uint64_t qubit, double u, int *outcome);
+
+/* FP64 observables (src/state.rs:94-185). */
+int ev_sv_norm_sqr(const ev_sv *s, double *out);                        /* state.rs:94-96 */
+int ev_sv_normalize(ev_sv *s);                                          /* state.rs:103-111 */
+int ev_sv_probability(const ev_sv *s, uint64_t basis, double *out);     /* state.rs:119-122 */
+int ev_sv_overlap(const ev_sv *s, const ev_sv *other, ev_c128 *out);    /* state.rs:130-138 */
+int ev_sv_fidelity(const ev_sv *s, const ev_sv *other, double *out);    /* state.rs:149-151 */
+int ev_sv_bloch_vector(const ev_sv *s, uint64_t qubit, double out[3]);  /* state.rs:163-185 */
+/* Born-rule sampling: `shots` basis indices, one Rng::next_f64 per shot from
+ * Rng::seed_from_u64(seed), inverted through the f64 inclusive CDF of |amp|^2. */
+int ev_sv_sample(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
+
+/* Execution options (not in the reference; defaults = zero-initialised). */
+typedef struct {
+    ev_precision precision; /* storage precision of the run */
+    uint32_t plain;         /* 1: one launch per op (unfused); 0: planned fused passes */
+    uint32_t tile_qubits;   /* fused pass tile width; 0 = default */
+    uint32_t max_gates_per_pass; /* 0 = default */
+    uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
+    uint32_t reserved[3];
+} ev_run_options;
+
+/* src/backend/statevector.rs:68-88: validate, |0..0>, Rng::seed_from_u64(seed),
+ * drive the ops.  On success *out_state owns the final state and classical[]
+ * (num_classical bytes, 0/1) holds the classical register.  On a validation
+ * failure *err (may be NULL) receives the first error found. */
+int ev_sv_run(const ev_circuit *c, uint64_t seed, const ev_run_options *opt, ev_sv **out_state,
+              uint8_t *classical, size_t num_classical, ev_error *err);
+
+/* Same as ev_sv_run but on an existing state, which is first reset to
+ * |0..0> (State::zero without a new allocation). */
+int ev_sv_run_into(ev_sv *s, const ev_circuit *c, uint64_t seed, const ev_run_options *opt, uint8_t *classical,
+                   size_t num_classical, ev_error *err);
+/* Resets s to |0..0> in place. */
+int ev_sv_reset(ev_sv *s);
+
+/* Host-slice shims with the exact signatures of src/kernel.rs's pub fns.
+ * They copy `amps` to the device, apply one gate and copy back. */
+int ev_kernel_apply_1q(ev_c128 *amps, size_t len, size_t k, const ev_c128 m[4]);
+int ev_kernel_apply_2q(ev_c128 *amps, size_t len, size_t a, size_t b, const ev_c128 m[16]);
+int ev_kernel_apply_controlled_1q(ev_c128 *amps, size_t len, const uint64_t *controls, size_t num_controls,
+                                  size_t target, const ev_c128 m[4]);
+
+/* ------------------------------------------------------- density matrix */
+
+/* src/noise.rs:160-174 Channel; kraus/num_kraus are used by EV_CH_CUSTOM
+ * (2x2 row-major operators, 4 entries each). */
+typedef enum {
+    EV_CH_NONE = 0,
+    EV_CH_DEPOLARIZING = 1,
+    EV_CH_AMPLITUDE_DAMPING = 2,
+    EV_CH_PHASE_DAMPING = 3,
+    EV_CH_BIT_FLIP = 4,
+    EV_CH_DEPHASING = 5,
+    EV_CH_CUSTOM = 6
+} ev_channel_kind;
+
+typedef struct {
+    int32_t kind;
+    double p;
+    const ev_c128 *kraus;
+    size_t num_kraus;
+} ev_channel;
+
+/* src/noise.rs:31-40 NoiseModel { after_1q, after_2q, readout_error } */
+typedef struct {
+    ev_channel after_1q;
+    ev_channel after_2q;
+    double readout_error;
+} ev_noise_model;
+
+/* src/noise.rs:179-234 Channel::kraus_ops: writes up to cap operators. */
+int ev_channel_kraus(const ev_channel *ch, ev_c128 *out, size_t cap, size_t *count);
+
+/* Device-resident density matrix (src/density.rs:31-36), row-major dim x dim. */
+typedef struct ev_dm ev_dm;
+int ev_dm_create(uint64_t n, ev_dm **out);                              /* density.rs:45-55 */
+void ev_dm_destroy(ev_dm *d);
+uint64_t ev_dm_num_qubits(const ev_dm *d);
+int ev_dm_read(const ev_dm *d, ev_c128 *host, size_t len);              /* density.rs:101-103 */
+int ev_dm_trace(const ev_dm *d, double *out);                           /* density.rs:121-124 */
+int ev_dm_purity(const ev_dm *d, double *out);                          /* density.rs:130-140 */
+int ev_dm_probability(const ev_dm *d, uint64_t basis, double *out);     /* density.rs:150-158 */
+int ev_dm_expectation_pauli(const ev_dm *d, uint64_t qubit, char pauli, double *out); /* density.rs:237-261 */
+int ev_dm_partial_trace_1q(const ev_dm *d, uint64_t qubit, ev_c128 out[4]);         /* density.rs:270-288 */
+
+/* src/backend/density_matrix.rs:97-113 run_with_noise_seeded.  noise may be
+ * NULL (NoiseModel::ideal, density_matrix.rs:76-78). */
+int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, const ev_run_options *opt,
+              ev_dm **out, uint8_t *classical, size_t num_classical, ev_error *err);
+
+/* ------------------------------------------------------------- profiling */
+
+/* Statistics of the most recent ev_sv_run / ev_dm_run on the calling thread. */
+typedef struct {
+    uint64_t logical_gates; /* gate ops applied (unfused count) */
+    uint64_t passes;        /* full-state kernel passes launched */
+    uint64_t launches;      /* kernel launches */
+    double device_ms;       /* device time of the op stream (HIP events) */
+    double pass_ms;         /* summed device time of the fused pass kernels */
+    uint64_t pass_launches; /* fused pass kernel launches */
+    double plan_ms;         /* host planning time */
+} ev_run_stats;
+int ev_last_run_stats(ev_run_stats *out);
+
+/* Per-kernel timing of the most recent run made with ev_run_options.profile
+ * set: launches, summed device time (HIP events on the handle's stream) and
+ * the algorithmic HBM bytes those launches move. */
+typedef struct {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+    double bytes;
+} ev_kernel_stat;
+int ev_last_run_kernel_stats(ev_kernel_stat *out, size_t cap, size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EVERETT_H */
