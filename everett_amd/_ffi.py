@@ -90,6 +90,14 @@ class EvKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
+class EvPlanInfo(C.Structure):
+    _fields_ = [
+        ("num_passes", C.c_uint32), ("num_stages", C.c_uint32), ("num_ops", C.c_uint32), ("num_coef", C.c_uint32),
+        ("pass_bytes", C.c_uint32), ("stage_bytes", C.c_uint32), ("op_bytes", C.c_uint32),
+        ("tile_qubits", C.c_uint32),
+    ]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 U64 = C.c_uint64
@@ -123,6 +131,7 @@ PROTOTYPES = {
     "ev_sv_precision": (I, [P]),
     "ev_sv_read": (I, [P, P, SZ]),
     "ev_sv_write": (I, [P, P, SZ]),
+    "ev_sv_read_range": (I, [P, U64, P, SZ]),
     "ev_sv_copy": (I, [P, PP]),
     "ev_sv_apply_1q": (I, [P, U64, P]),
     "ev_sv_apply_2q": (I, [P, U64, U64, P]),
@@ -154,6 +163,7 @@ PROTOTYPES = {
     "ev_dm_run": (I, [P, C.POINTER(EvNoiseModel), U64, C.POINTER(EvRunOptions), PP, P, SZ, C.POINTER(EvError)]),
     "ev_last_run_stats": (I, [C.POINTER(EvRunStats)]),
     "ev_last_run_kernel_stats": (I, [C.POINTER(EvKernelStat), SZ, C.POINTER(SZ)]),
+    "ev_debug_plan": (I, [P, C.POINTER(EvRunOptions), P, P, P, P, C.POINTER(EvPlanInfo)]),
 }
 
 _lib = None

@@ -11,6 +11,8 @@
 
 #include "dm_ops.hpp"
 #include "engine.hpp"
+#include "fused.hpp"
+#include "planner.hpp"
 
 struct ev_circuit {
     ev::Circuit c;
@@ -274,6 +276,15 @@ int ev_sv_read(const ev_sv *s, ev_c128 *host, size_t len)
         NONNULL(s);
         NONNULL(host);
         ev::sv_read(const_cast<ev_sv *>(s), reinterpret_cast<ev::c128 *>(host), len);
+    });
+}
+
+int ev_sv_read_range(const ev_sv *s, uint64_t offset, ev_c128 *host, size_t len)
+{
+    return guard([&] {
+        NONNULL(s);
+        EV_REQUIRE(len == 0 || host != nullptr, EV_E_ARG, "host is null");
+        ev::sv_read_range(const_cast<ev_sv *>(s), offset, reinterpret_cast<ev::c128 *>(host), len);
     });
 }
 
@@ -698,6 +709,31 @@ int ev_last_run_stats(ev_run_stats *out)
         out->pass_ms = s.pass_ms;
         out->pass_launches = s.pass_launches;
         out->plan_ms = s.plan_ms;
+    });
+}
+
+int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, void *stages, void *ops,
+                  ev_c128 *coef, ev_plan_info *info)
+{
+    return guard([&] {
+        NONNULL(c);
+        NONNULL(info);
+        c->c.validate();
+        ev_run_options o{};
+        if (opt) o = *opt;
+        const ev::PlanBlob b = ev::debug_plan(c->c, o);
+        info->num_passes = b.np;
+        info->num_stages = b.ns;
+        info->num_ops = b.no;
+        info->num_coef = uint32_t(b.coef.size());
+        info->pass_bytes = sizeof(ev::fused::Pass);
+        info->stage_bytes = sizeof(ev::fused::Stage);
+        info->op_bytes = sizeof(ev::fused::DevOp);
+        info->tile_qubits = b.t;
+        if (passes) std::memcpy(passes, b.passes.data(), b.passes.size());
+        if (stages) std::memcpy(stages, b.stages.data(), b.stages.size());
+        if (ops) std::memcpy(ops, b.ops.data(), b.ops.size());
+        if (coef) std::memcpy(coef, b.coef.data(), b.coef.size() * sizeof(ev_c128));
     });
 }
 

@@ -133,8 +133,6 @@ DmNoise make_dm_noise(const ev_noise_model *noise)
     return nz;
 }
 
-namespace {
-
 // backend/mod.rs:51-72 over the density-matrix primitives, one pass per
 // superoperator.
 size_t dm_plain_op(ev_dm *d, const Circuit &c, size_t i, const DmNoise &nz, Rng &rng, uint8_t *classical, bool execute,
@@ -190,8 +188,6 @@ size_t dm_plain_op(ev_dm *d, const Circuit &c, size_t i, const DmNoise &nz, Rng 
     }
     throw Error(EV_E_ARG, "corrupt op record");
 }
-
-}  // namespace
 
 void dm_execute(ev_dm *d, const Circuit &c, const ev_noise_model *noise, Rng &rng, uint8_t *classical,
                 const ev_run_options &opt)

@@ -31,6 +31,11 @@ void dm_partial_trace(ev_dm *d, int q, c128 out[4]);
 void dm_renormalize(ev_dm *d);
 bool dm_measure(ev_dm *d, int q, Rng &rng, double readout_error);
 
+// backend/mod.rs:51-72 over the density-matrix primitives for record i, one
+// launch per (super)operator; returns the index after the op.
+size_t dm_plain_op(ev_dm *d, const Circuit &c, size_t i, const DmNoise &nz, Rng &rng, uint8_t *classical, bool execute,
+                   RunStats &st);
+
 // Fused execution of a density-matrix circuit (planner.cpp).
 void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng, uint8_t *classical,
                         const ev_run_options &opt, RunStats &st);

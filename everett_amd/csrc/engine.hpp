@@ -15,6 +15,8 @@
 // the amplitudes in HBM.
 struct ev_sv {
     int n = 0;
+    void *plan_buf = nullptr;       // device copy of the fused pass descriptors
+    size_t plan_cap = 0;
     ev::kern::Prec prec = ev::kern::C128;
     void *psi = nullptr;            // 2^n amplitudes
     hipStream_t stream = nullptr;   // all work on this handle is ordered here
@@ -51,6 +53,7 @@ void sv_bloch(ev_sv *s, int q, double out[4]);
 bool sv_measure(ev_sv *s, int q, double u);
 void sv_normalize(ev_sv *s);
 void sv_read(ev_sv *s, c128 *host, uint64_t len);
+void sv_read_range(ev_sv *s, uint64_t offset, c128 *host, uint64_t len);
 void sv_write(ev_sv *s, const c128 *host, uint64_t len);
 void sv_sample(ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
 
@@ -99,6 +102,11 @@ const char *k2_name(K2 k);
 double k1_bytes(K1 k, int n, size_t s);
 double k2_bytes(K2 k, int n, size_t s);
 inline double ctrl_bytes(int n, int nc, size_t s) { return 2.0 * double(uint64_t(1) << (n - nc)) * double(s); }
+
+// backend/mod.rs:51-72 for record i with one launch per gate; returns the
+// index after the op (an If record consumes the op it guards).
+size_t sv_plain_op(ev_sv *s, const Circuit &c, size_t i, Rng &rng, uint8_t *classical, bool execute, RunStats &st,
+                   Profiler &prof);
 
 // Resets s to |0..0> and drives c on it (validation is the caller's job).
 void sv_run_into(ev_sv *s, const Circuit &c, uint64_t seed, uint8_t *classical, const ev_run_options &opt);

@@ -1,0 +1,80 @@
+// Descriptors of a fused statevector pass, shared by the planner (host) and
+// the pass kernel (device).
+//
+// A pass picks t "tile" qubits Q (always including the lowest qubits, so each
+// tile row is a contiguous run of amplitudes).  Workgroup w owns the 2^t
+// amplitudes whose non-tile bits spell w; it streams them from HBM into LDS,
+// runs the pass program, and streams them back: one HBM read and one write of
+// the whole state per pass, whatever the number of gates in it.
+//
+// The program is a list of stages.  In a stage each of the 2^(t-4) threads
+// holds 16 amplitudes in registers, whose local index bits at the stage's 4
+// "register" positions vary and whose other bits spell the thread id; the
+// stage's ops act on those registers (targets must be register bits, controls
+// and diagonal factors may depend on any bit of the global index).  Between
+// stages the tile goes back to LDS and is re-read with the next mapping.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define EV_HOST_DEVICE __host__ __device__
+#else
+#define EV_HOST_DEVICE
+#endif
+
+namespace ev::fused {
+
+constexpr int kRegBits = 4;
+constexpr int kRegs = 1 << kRegBits;  // amplitudes per thread per stage
+constexpr int kMaxTile = 13;          // 2^13 x 8 B (c64) or 2^12 x 16 B (c128) = 64 KiB of LDS
+constexpr int kMinTile = 8;
+
+enum OpKind : uint8_t {
+    kU1 = 0,     // dense 2x2 on register bit b0
+    kCU1 = 1,    // dense 2x2 on register bit b0 where the control condition holds
+    kCX = 2,     // swap register pairs across bit b0 where the condition holds (CNOT, multi-controlled X)
+    kDiagR = 3,  // diag(d0, d1) on register bit b0 where the condition holds
+    kDiagT = 4,  // diag(d0, d1) on global qubit tq (not a register bit) where the condition holds
+    kU2 = 5,     // dense 4x4 on register bits (b0 = MSB of the gate basis, b1 = LSB), b0 > b1
+    kSwap = 6,   // exchange register bits b0 > b1
+};
+
+enum OpFlags : uint8_t { kD0IsOne = 1 };
+
+struct DevOp {
+    uint8_t kind, b0, b1, flags;
+    uint32_t creg;   // condition on the register index j: (j & creg) == creg
+    uint64_t cglob;  // condition on the thread's fixed global index bits
+    uint32_t coef;   // offset (complex entries) into the coefficient pool
+    uint32_t tq;     // kDiagT target qubit
+};
+
+struct Stage {
+    int32_t op_begin, op_end;
+    int32_t np;               // thread-id bits (t - 4)
+    int32_t pad;
+    uint32_t roff[kRegs];     // swizzled LDS slot offset of register j
+    uint64_t goff[kRegs];     // global index offset of register j
+    int8_t p[kMaxTile];       // thread-id bit i -> local position p[i]
+    int8_t gpos[kMaxTile];    // thread-id bit i -> global qubit
+    int8_t pad2[6];
+};
+
+struct Pass {
+    int32_t t;
+    int32_t nstages;
+    int32_t stage_begin;
+    int32_t ncomp;            // n - t
+    int8_t q[16];             // local bit -> global qubit (ascending)
+    int8_t comp[40];          // tile-index bit -> global qubit (the non-tile qubits, ascending)
+};
+
+// LDS slot of local index l: the low 4 bits are XORed with the higher
+// nibbles, so 16 consecutive lanes land in 16 distinct 16-byte slots of a
+// 256-byte LDS row for every register mapping whose varying lane bits have
+// distinct positions mod 4.  Linear over XOR, hence swz(a | b) = swz(a) ^ swz(b)
+// for disjoint a, b.
+EV_HOST_DEVICE inline uint32_t swz(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8) ^ (l >> 12)) & 15u); }
+
+}  // namespace ev::fused

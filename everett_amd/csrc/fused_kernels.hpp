@@ -1,0 +1,16 @@
+// Launcher of the fused statevector pass kernel (fused_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "fused.hpp"
+
+namespace ev::fused {
+
+// One launch = one pass over a 2^n state with tile width t: 2^(n-t)
+// workgroups of 2^(t-4) threads.  Descriptor pointers are device memory;
+// coef holds double2 (c128) or float2 (c64) entries.
+void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
+                 const void *coef, hipStream_t st);
+
+}  // namespace ev::fused

@@ -1,18 +1,599 @@
-// Fused pass planner and executor (statevector and density matrix).
+// Fused execution: the gate planner and the pass executor.
+//
+// 1. Lowering.  Circuit ops become planner ops (GOp).  Consecutive one-qubit
+//    gates on a qubit are multiplied into one 2x2 in f64 (gate::compose), a
+//    diagonal product becomes a phase op that needs no tile residency, CNOT
+//    and multi-controlled X become register swaps, CZ and controlled phases
+//    become conditional diagonals.
+// 2. Passes.  Greedy in program order: a pass owns a set Q of t tile qubits
+//    (always containing the lowest qubits); an op joins the pass if the qubits
+//    it mixes fit in Q, none of its qubits was touched by an op already
+//    deferred, and the pass's compute budget allows; otherwise it is deferred.
+//    Ops on disjoint qubits commute, so the deferred ones keep their order
+//    relative to everything they touch.
+// 3. Stages.  The same greedy rule inside a pass with 4 register qubits.
+//
+// The result runs as one kernel launch per pass (fused_kernels.hip).
 #include "planner.hpp"
 
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
 #include "dm_ops.hpp"
+#include "fused.hpp"
+#include "fused_kernels.hpp"
 
 namespace ev {
 
-void execute_planned(ev_sv *, const Circuit &, Rng &, uint8_t *, const ev_run_options &, RunStats &, Profiler &)
+namespace {
+
+using fused::DevOp;
+using fused::kRegBits;
+using fused::kRegs;
+using fused::Pass;
+using fused::Stage;
+
+enum GKind : uint8_t { G_U1, G_CU1, G_CX, G_DIAG, G_U2, G_SWAP };
+
+struct GOp {
+    GKind kind;
+    int q0, q1;      // target(s); U2: q0 = MSB of the gate basis
+    uint64_t cmask;  // control qubits
+    uint32_t coef;   // offset into the c128 pool (U1/CU1: 4, DIAG: 2, U2: 16 entries)
+    float cost;
+    uint64_t touched() const
+    {
+        uint64_t m = cmask | (uint64_t(1) << q0);
+        if (kind == G_U2 || kind == G_SWAP) m |= uint64_t(1) << q1;
+        return m;
+    }
+    // Qubits that must be tile (and register) qubits when the op runs.
+    uint64_t need() const
+    {
+        switch (kind) {
+        case G_DIAG: return 0;
+        case G_U2:
+        case G_SWAP: return (uint64_t(1) << q0) | (uint64_t(1) << q1);
+        default: return uint64_t(1) << q0;
+        }
+    }
+};
+
+// Compute cost in units of one dense 2x2 over the tile.
+float op_cost(GKind k, bool d0_one)
 {
-    throw Error(EV_E_STATE, "fused execution is not available in this build; set ev_run_options.plain = 1");
+    switch (k) {
+    case G_U1:
+    case G_CU1: return 1.0f;
+    case G_U2: return 2.0f;
+    case G_DIAG: return d0_one ? 0.25f : 0.5f;
+    default: return 0.1f;
+    }
 }
 
-void execute_planned_dm(ev_dm *, const Circuit &, const DmNoise &, Rng &, uint8_t *, const ev_run_options &, RunStats &)
+// Lowers circuit ops into GOps, merging one-qubit runs per qubit.
+class Lowering {
+  public:
+    explicit Lowering(int n) : pend_(n), has_(n, false) {}
+
+    std::vector<GOp> ops;
+    std::vector<c128> pool;
+    uint64_t logical = 0;
+
+    void gate1(int q, const c128 *m)
+    {
+        const Mat2 g{m[0], m[1], m[2], m[3]};
+        pend_[q] = has_[q] ? gate::compose(g, pend_[q]) : g;
+        has_[q] = true;
+    }
+
+    void flush(int q)
+    {
+        if (!has_[q]) return;
+        has_[q] = false;
+        const Mat2 &m = pend_[q];
+        const Mat2 id = gate::id();
+        bool is_id = true;
+        for (int i = 0; i < 4; ++i) is_id = is_id && ceq(m[i], id[i]);
+        if (is_id) return;
+        if (classify1(m.data()) != K1::Dense) {
+            diag(q, 0, m[0], m[3]);
+        } else {
+            push(G_U1, q, 0, 0, m.data(), 4);
+        }
+    }
+
+    void flush_all()
+    {
+        for (int q = 0; q < int(has_.size()); ++q) flush(q);
+    }
+
+    void gate2(int a, int b, const c128 *m)
+    {
+        flush(a);
+        flush(b);
+        switch (classify2(m)) {
+        case K2::Cnot: push(G_CX, b, 0, uint64_t(1) << a, nullptr, 0); break;
+        case K2::Cz: diag(b, uint64_t(1) << a, c128{1, 0}, c128{-1, 0}); break;
+        case K2::Swap: push(G_SWAP, std::max(a, b), std::min(a, b), 0, nullptr, 0); break;
+        case K2::Dense: push(G_U2, a, b, 0, m, 16); break;
+        }
+    }
+
+    void controlled(const uint64_t *cs, int nc, int t, const c128 *m)
+    {
+        flush(t);
+        uint64_t mask = 0;
+        for (int i = 0; i < nc; ++i) {
+            flush(int(cs[i]));
+            mask |= uint64_t(1) << cs[i];
+        }
+        const Mat2 x = gate::x();
+        bool is_x = true;
+        for (int i = 0; i < 4; ++i) is_x = is_x && ceq(m[i], x[i]);
+        if (classify1(m) != K1::Dense) {
+            diag(t, mask, m[0], m[3]);
+        } else if (is_x) {
+            push(G_CX, t, 0, mask, nullptr, 0);
+        } else {
+            push(mask ? G_CU1 : G_U1, t, 0, mask, m, 4);
+        }
+    }
+
+    // Row/column superoperator (DM): dense 4x4 on (a = MSB, b).
+    void dense2(int a, int b, const c128 *m)
+    {
+        flush(a);
+        flush(b);
+        push(G_U2, a, b, 0, m, 16);
+    }
+
+  private:
+    void diag(int q, uint64_t mask, c128 d0, c128 d1)
+    {
+        GOp g{G_DIAG, q, 0, mask, uint32_t(pool.size()), 0};
+        pool.push_back(d0);
+        pool.push_back(d1);
+        g.cost = op_cost(G_DIAG, d0.re == 1 && d0.im == 0);
+        ops.push_back(g);
+    }
+    void push(GKind k, int q0, int q1, uint64_t mask, const c128 *m, int nm)
+    {
+        GOp g{k, q0, q1, mask, uint32_t(pool.size()), op_cost(k, false)};
+        if (nm) pool.insert(pool.end(), m, m + nm);
+        ops.push_back(g);
+    }
+    std::vector<Mat2> pend_;
+    std::vector<bool> has_;
+};
+
+struct Plan {
+    std::vector<Pass> passes;
+    std::vector<Stage> stages;
+    std::vector<DevOp> devops;
+    std::vector<c128> coef;
+};
+
+int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+int rank_in(uint64_t mask, int bit) { return popc(mask & ((uint64_t(1) << bit) - 1)); }
+
+// Builds the pass list for `g` on an n-qubit state.
+Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t, int low, float budget)
 {
-    throw Error(EV_E_STATE, "fused execution is not available in this build; set ev_run_options.plain = 1");
+    Plan P;
+    const uint64_t all = (n >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
+    const uint64_t lowmask = (uint64_t(1) << low) - 1;
+    std::vector<int> rem(g.size());
+    for (size_t i = 0; i < g.size(); ++i) rem[i] = int(i);
+
+    while (!rem.empty()) {
+        uint64_t Q = lowmask, blocked = 0;
+        float cost = 0;
+        std::vector<int> take, defer;
+        size_t i = 0;
+        for (; i < rem.size(); ++i) {
+            const GOp &op = g[rem[i]];
+            const uint64_t tq = op.touched();
+            if (tq & blocked) {
+                blocked |= tq;
+                defer.push_back(rem[i]);
+            } else {
+                const uint64_t nq = Q | op.need();
+                if (popc(nq) <= t && (take.empty() || cost + op.cost <= budget)) {
+                    Q = nq;
+                    cost += op.cost;
+                    take.push_back(rem[i]);
+                } else {
+                    blocked |= tq;
+                    defer.push_back(rem[i]);
+                }
+            }
+            if ((blocked & all) == all) {
+                ++i;
+                break;
+            }
+        }
+        for (; i < rem.size(); ++i) defer.push_back(rem[i]);
+        rem.swap(defer);
+
+        // Pad Q to exactly t qubits with the lowest unused ones.
+        for (int q = 0; q < n && popc(Q) < t; ++q) Q |= uint64_t(1) << q;
+
+        Pass ps{};
+        ps.t = t;
+        ps.ncomp = n - t;
+        int k = 0, c = 0;
+        for (int q = 0; q < n; ++q) {
+            if ((Q >> q) & 1) ps.q[k++] = int8_t(q);
+            else ps.comp[c++] = int8_t(q);
+        }
+        ps.stage_begin = int(P.stages.size());
+        auto local = [&](int q) { return rank_in(Q, q); };
+
+        // Stages: greedy with 4 register qubits (local positions).
+        std::vector<int> srem = take;
+        while (!srem.empty()) {
+            uint32_t S = 0;
+            uint64_t sblocked = 0;
+            std::vector<int> stake, sdefer;
+            for (int id : srem) {
+                const GOp &op = g[id];
+                const uint64_t tq = op.touched();
+                if (tq & sblocked) {
+                    sblocked |= tq;
+                    sdefer.push_back(id);
+                    continue;
+                }
+                uint32_t need = 0;
+                for (uint64_t m = op.need(); m; m &= m - 1) need |= 1u << local(__builtin_ctzll(m));
+                if (popc(S | need) <= kRegBits) {
+                    S |= need;
+                    stake.push_back(id);
+                } else {
+                    sblocked |= tq;
+                    sdefer.push_back(id);
+                }
+            }
+            srem.swap(sdefer);
+            for (int l = 0; l < t && popc(S) < kRegBits; ++l) S |= 1u << l;
+
+            Stage st{};
+            st.op_begin = int(P.devops.size());
+            st.np = t - kRegBits;
+            int sp[kRegBits], ns = 0, np = 0;
+            for (int l = 0; l < t; ++l) {
+                if ((S >> l) & 1) sp[ns++] = l;
+                else {
+                    st.p[np] = int8_t(l);
+                    st.gpos[np] = ps.q[l];
+                    ++np;
+                }
+            }
+            for (int j = 0; j < kRegs; ++j) {
+                uint32_t off = 0;
+                uint64_t goff = 0;
+                for (int b = 0; b < kRegBits; ++b)
+                    if ((j >> b) & 1) {
+                        off |= 1u << sp[b];
+                        goff |= uint64_t(1) << ps.q[sp[b]];
+                    }
+                st.roff[j] = fused::swz(off);
+                st.goff[j] = goff;
+            }
+            // register bit of global qubit q (or -1)
+            auto regbit = [&](int q) -> int {
+                if (!((Q >> q) & 1)) return -1;
+                const int l = local(q);
+                if (!((S >> l) & 1)) return -1;
+                return rank_in(S, l);
+            };
+            for (int id : stake) {
+                const GOp &op = g[id];
+                DevOp d{};
+                for (uint64_t m = op.cmask; m; m &= m - 1) {
+                    const int q = __builtin_ctzll(m);
+                    const int rb = regbit(q);
+                    if (rb >= 0) d.creg |= 1u << rb;
+                    else d.cglob |= uint64_t(1) << q;
+                }
+                d.coef = uint32_t(P.coef.size());
+                switch (op.kind) {
+                case G_U1:
+                case G_CU1:
+                    d.kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
+                    d.b0 = uint8_t(regbit(op.q0));
+                    P.coef.insert(P.coef.end(), pool.begin() + op.coef, pool.begin() + op.coef + 4);
+                    break;
+                case G_CX:
+                    d.kind = fused::kCX;
+                    d.b0 = uint8_t(regbit(op.q0));
+                    break;
+                case G_DIAG: {
+                    const c128 d0 = pool[op.coef], d1 = pool[op.coef + 1];
+                    const int rb = regbit(op.q0);
+                    if (rb >= 0) {
+                        d.kind = fused::kDiagR;
+                        d.b0 = uint8_t(rb);
+                    } else {
+                        d.kind = fused::kDiagT;
+                        d.tq = uint32_t(op.q0);
+                    }
+                    if (d0.re == 1 && d0.im == 0) d.flags |= fused::kD0IsOne;
+                    P.coef.push_back(d0);
+                    P.coef.push_back(d1);
+                    break;
+                }
+                case G_U2: {
+                    d.kind = fused::kU2;
+                    const int ra = regbit(op.q0), rb = regbit(op.q1);
+                    const c128 *m = pool.data() + op.coef;
+                    if (ra > rb) {
+                        d.b0 = uint8_t(ra);
+                        d.b1 = uint8_t(rb);
+                        P.coef.insert(P.coef.end(), m, m + 16);
+                    } else {  // reorder the basis |q_a q_b> -> |q_b q_a>
+                        d.b0 = uint8_t(rb);
+                        d.b1 = uint8_t(ra);
+                        auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
+                        c128 mm[16];
+                        for (int r = 0; r < 4; ++r)
+                            for (int cc = 0; cc < 4; ++cc) mm[sw(r) * 4 + sw(cc)] = m[r * 4 + cc];
+                        P.coef.insert(P.coef.end(), mm, mm + 16);
+                    }
+                    break;
+                }
+                case G_SWAP: {
+                    d.kind = fused::kSwap;
+                    const int ra = regbit(op.q0), rb = regbit(op.q1);
+                    d.b0 = uint8_t(std::max(ra, rb));
+                    d.b1 = uint8_t(std::min(ra, rb));
+                    break;
+                }
+                }
+                P.devops.push_back(d);
+            }
+            st.op_end = int(P.devops.size());
+            P.stages.push_back(st);
+        }
+        ps.nstages = int(P.stages.size()) - ps.stage_begin;
+        P.passes.push_back(ps);
+    }
+    return P;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct FusedConfig {
+    int t, low;
+    float budget;
+};
+
+FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
+{
+    FusedConfig f;
+    const int tmax = c64 ? fused::kMaxTile : fused::kMaxTile - 1;  // 64 KiB of LDS per tile
+    f.t = opt.tile_qubits ? int(opt.tile_qubits) : tmax;
+    EV_REQUIRE(f.t >= fused::kMinTile && f.t <= tmax, EV_E_ARG,
+               "tile_qubits must be in [8, " + std::to_string(tmax) + "] for this precision");
+    f.t = std::min(f.t, n);
+    f.low = c64 ? 5 : 4;  // 256-byte contiguous rows
+    f.budget = opt.max_gates_per_pass ? float(opt.max_gates_per_pass) : 20.0f;
+    return f;
+}
+
+// Uploads a plan and launches its passes on s's stream.
+void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
+{
+    if (P.passes.empty()) return;
+    const bool c64 = s->prec == kern::C64;
+    const size_t ab = kern::amp_bytes(s->prec);
+    const size_t o_pass = 0;
+    const size_t o_stage = align256(P.passes.size() * sizeof(Pass));
+    const size_t o_ops = o_stage + align256(P.stages.size() * sizeof(Stage));
+    const size_t o_coef = o_ops + align256(std::max<size_t>(1, P.devops.size()) * sizeof(DevOp));
+    const size_t total = o_coef + align256(std::max<size_t>(1, P.coef.size()) * ab);
+    std::vector<unsigned char> host(total, 0);
+    std::memcpy(host.data() + o_pass, P.passes.data(), P.passes.size() * sizeof(Pass));
+    std::memcpy(host.data() + o_stage, P.stages.data(), P.stages.size() * sizeof(Stage));
+    if (!P.devops.empty()) std::memcpy(host.data() + o_ops, P.devops.data(), P.devops.size() * sizeof(DevOp));
+    if (!c64) {
+        std::memcpy(host.data() + o_coef, P.coef.data(), P.coef.size() * sizeof(c128));
+    } else {
+        float *f = reinterpret_cast<float *>(host.data() + o_coef);
+        for (size_t i = 0; i < P.coef.size(); ++i) {
+            f[2 * i] = float(P.coef[i].re);
+            f[2 * i + 1] = float(P.coef[i].im);
+        }
+    }
+    if (s->plan_cap < total) {
+        sv_sync(s);
+        if (s->plan_buf) EV_HIP(hipFree(s->plan_buf));
+        s->plan_buf = nullptr;
+        s->plan_cap = 0;
+        EV_HIP(hipMalloc(&s->plan_buf, total));
+        s->plan_cap = total;
+    }
+    unsigned char *dev = static_cast<unsigned char *>(s->plan_buf);
+    EV_HIP(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, s->stream));
+    EV_HIP(hipStreamSynchronize(s->stream));  // the host staging buffer is released on return
+    const Pass *dp = reinterpret_cast<const Pass *>(dev + o_pass);
+    const Stage *ds = reinterpret_cast<const Stage *>(dev + o_stage);
+    const DevOp *dops = reinterpret_cast<const DevOp *>(dev + o_ops);
+    const void *dcoef = dev + o_coef;
+    for (size_t p = 0; p < P.passes.size(); ++p) {
+        prof.begin();
+        fused::launch_pass(s->psi, c64, s->n, P.passes[p].t, dp + p, ds, dops, dcoef, s->stream);
+        prof.end("fused_pass", full_pass_bytes(s->n, ab));
+        ++st.passes;
+        ++st.pass_launches;
+        ++st.launches;
+    }
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st, Profiler &prof)
+{
+    low.flush_all();
+    if (low.ops.empty()) return;
+    const double t0 = now_ms();
+    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, cfg.budget);
+    st.plan_ms += now_ms() - t0;
+    run_plan(s, P, st, prof);
+    low.ops.clear();
+    low.pool.clear();
+}
+
+}  // namespace
+
+void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const ev_run_options &opt,
+                     RunStats &st, Profiler &prof)
+{
+    if (s->n < fused::kMinTile) {  // below one tile: per-gate kernels
+        size_t i = 0;
+        while (i < c.size()) i = sv_plain_op(s, c, i, rng, classical, true, st, prof);
+        return;
+    }
+    const FusedConfig cfg = fused_config(s->n, s->prec == kern::C64, opt);
+    Lowering low(s->n);
+    size_t i = 0;
+    while (i < c.size()) {
+        const Op &op = c.ops()[i];
+        switch (op.kind) {
+        case OpKind::Apply1:
+            low.gate1(int(op.q0), c.mat(op));
+            ++st.logical_gates;
+            ++i;
+            break;
+        case OpKind::Apply2:
+            low.gate2(int(op.q0), int(op.q1), c.mat(op));
+            ++st.logical_gates;
+            ++i;
+            break;
+        case OpKind::Controlled:
+            low.controlled(c.ctrls(op), int(op.nctrl), int(op.q0), c.mat(op));
+            ++st.logical_gates;
+            ++i;
+            break;
+        case OpKind::Measure:
+        case OpKind::If:
+            flush_segment(s, low, cfg, st, prof);
+            i = sv_plain_op(s, c, i, rng, classical, true, st, prof);
+            break;
+        }
+    }
+    flush_segment(s, low, cfg, st, prof);
+}
+
+PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)
+{
+    const int n = int(c.num_qubits());
+    EV_REQUIRE(n >= fused::kMinTile, EV_E_ARG, "fused plans need at least 8 qubits");
+    const FusedConfig cfg = fused_config(n, opt.precision == EV_C64, opt);
+    Lowering low(n);
+    for (size_t i = 0; i < c.size(); ++i) {
+        const Op &op = c.ops()[i];
+        switch (op.kind) {
+        case OpKind::Apply1: low.gate1(int(op.q0), c.mat(op)); break;
+        case OpKind::Apply2: low.gate2(int(op.q0), int(op.q1), c.mat(op)); break;
+        case OpKind::Controlled: low.controlled(c.ctrls(op), int(op.nctrl), int(op.q0), c.mat(op)); break;
+        default: throw Error(EV_E_ARG, "debug_plan takes measurement-free circuits");
+        }
+    }
+    low.flush_all();
+    const Plan P = plan(low.ops, low.pool, n, cfg.t, cfg.low, cfg.budget);
+    PlanBlob b;
+    auto bytes = [](const void *p, size_t len) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        return std::vector<unsigned char>(c, c + len);
+    };
+    b.passes = bytes(P.passes.data(), P.passes.size() * sizeof(Pass));
+    b.stages = bytes(P.stages.data(), P.stages.size() * sizeof(Stage));
+    b.ops = bytes(P.devops.data(), P.devops.size() * sizeof(DevOp));
+    b.coef = P.coef;
+    b.np = uint32_t(P.passes.size());
+    b.ns = uint32_t(P.stages.size());
+    b.no = uint32_t(P.devops.size());
+    b.t = uint32_t(cfg.t);
+    return b;
+}
+
+// Density matrix as a 2n-qubit vector (element (i, j) at (i << n) | j):
+// U rho U^dag = U on row bit n+t and conj(U) on column bit t; a channel is
+// the 4x4 superoperator on (n+t, t).
+void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng, uint8_t *classical,
+                        const ev_run_options &opt, RunStats &st)
+{
+    ev_sv *v = d->vec;
+    const int n = d->n;
+    Profiler prof(v->stream, false);
+    if (2 * n < fused::kMinTile) {
+        size_t i = 0;
+        while (i < c.size()) i = dm_plain_op(d, c, i, nz, rng, classical, true, st);
+        return;
+    }
+    const FusedConfig cfg = fused_config(2 * n, false, opt);
+    Lowering low(2 * n);
+    size_t i = 0;
+    while (i < c.size()) {
+        const Op &op = c.ops()[i];
+        switch (op.kind) {
+        case OpKind::Apply1: {  // density_matrix.rs:117-125
+            const c128 *m = c.mat(op);
+            const int t = int(op.q0);
+            if (nz.k1.empty()) {
+                low.gate1(n + t, m);
+                low.gate1(t, conj2(m).data());
+            } else {
+                low.dense2(n + t, t, gate_channel_superop(Mat2{m[0], m[1], m[2], m[3]}, nz.k1).data());
+            }
+            ++st.logical_gates;
+            ++i;
+            break;
+        }
+        case OpKind::Apply2: {  // density_matrix.rs:127-138
+            const int a = int(op.q0), b = int(op.q1);
+            low.gate2(n + a, n + b, c.mat(op));
+            low.gate2(a, b, conj4(c.mat(op)).data());
+            if (!nz.k2.empty()) {
+                low.dense2(n + a, a, nz.s2.data());
+                low.dense2(n + b, b, nz.s2.data());
+            }
+            ++st.logical_gates;
+            ++i;
+            break;
+        }
+        case OpKind::Controlled: {  // density_matrix.rs:140-153
+            const uint64_t *cs = c.ctrls(op);
+            std::vector<uint64_t> rows(cs, cs + op.nctrl);
+            for (auto &q : rows) q += uint64_t(n);
+            const int t = int(op.q0);
+            low.controlled(rows.data(), int(op.nctrl), n + t, c.mat(op));
+            low.controlled(cs, int(op.nctrl), t, conj2(c.mat(op)).data());
+            if (!nz.k1.empty()) {
+                low.dense2(n + t, t, nz.s1.data());
+                for (uint32_t k = 0; k < op.nctrl; ++k) low.dense2(n + int(cs[k]), int(cs[k]), nz.s1.data());
+            }
+            ++st.logical_gates;
+            ++i;
+            break;
+        }
+        case OpKind::Measure:
+        case OpKind::If:
+            flush_segment(v, low, cfg, st, prof);
+            i = dm_plain_op(d, c, i, nz, rng, classical, true, st);
+            break;
+        }
+    }
+    flush_segment(v, low, cfg, st, prof);
 }
 
 }  // namespace ev

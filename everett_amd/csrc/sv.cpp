@@ -57,6 +57,7 @@ void sv_destroy(ev_sv *s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->psi) (void)hipFree(s->psi);
     if (s->d_red) (void)hipFree(s->d_red);
+    if (s->plan_buf) (void)hipFree(s->plan_buf);
     if (s->h_red) (void)hipHostFree(s->h_red);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -132,6 +133,26 @@ void sv_read(ev_sv *s, c128 *host, uint64_t len)
     void *tmp = nullptr;
     EV_HIP(hipMalloc(&tmp, len * sizeof(c128)));
     kern::c64_to_c128(s->psi, tmp, len, s->stream);
+    const hipError_t e = hipMemcpyAsync(host, tmp, len * sizeof(c128), hipMemcpyDeviceToHost, s->stream);
+    const hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(tmp);
+    EV_HIP(e);
+    EV_HIP(e2);
+}
+
+void sv_read_range(ev_sv *s, uint64_t offset, c128 *host, uint64_t len)
+{
+    EV_REQUIRE(offset <= s->len() && len <= s->len() - offset, EV_E_ARG, "amplitude range out of bounds");
+    if (len == 0) return;
+    if (s->prec == kern::C128) {
+        EV_HIP(hipMemcpyAsync(host, static_cast<const char *>(s->psi) + offset * 16, len * 16, hipMemcpyDeviceToHost,
+                              s->stream));
+        sv_sync(s);
+        return;
+    }
+    void *tmp = nullptr;
+    EV_HIP(hipMalloc(&tmp, len * sizeof(c128)));
+    kern::c64_to_c128(static_cast<const char *>(s->psi) + offset * 8, tmp, len, s->stream);
     const hipError_t e = hipMemcpyAsync(host, tmp, len * sizeof(c128), hipMemcpyDeviceToHost, s->stream);
     const hipError_t e2 = hipStreamSynchronize(s->stream);
     (void)hipFree(tmp);
@@ -307,8 +328,10 @@ struct EventTimer {
     }
 };
 
+}  // namespace
+
 // backend/mod.rs:51-72, one launch per gate.
-size_t plain_op(ev_sv *s, const Circuit &c, size_t i, Rng &rng, uint8_t *classical, bool execute, RunStats &st,
+size_t sv_plain_op(ev_sv *s, const Circuit &c, size_t i, Rng &rng, uint8_t *classical, bool execute, RunStats &st,
                 Profiler &prof)
 {
     const Op &op = c.ops()[i];
@@ -346,12 +369,10 @@ size_t plain_op(ev_sv *s, const Circuit &c, size_t i, Rng &rng, uint8_t *classic
     case OpKind::Measure:
         if (execute) classical[op.q1] = sv_measure(s, int(op.q0), rng.next_f64()) ? 1 : 0;
         return i + 1;
-    case OpKind::If: return plain_op(s, c, i + 1, rng, classical, execute && classical[op.q0] != 0, st, prof);
+    case OpKind::If: return sv_plain_op(s, c, i + 1, rng, classical, execute && classical[op.q0] != 0, st, prof);
     }
     throw Error(EV_E_ARG, "corrupt op record");
 }
-
-}  // namespace
 
 void sv_execute(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const ev_run_options &opt)
 {
@@ -360,7 +381,7 @@ void sv_execute(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const 
     EventTimer timer(s->stream);
     if (opt.plain) {
         size_t i = 0;
-        while (i < c.size()) i = plain_op(s, c, i, rng, classical, true, st, prof);
+        while (i < c.size()) i = sv_plain_op(s, c, i, rng, classical, true, st, prof);
     } else {
         execute_planned(s, c, rng, classical, opt, st, prof);
     }

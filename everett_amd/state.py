@@ -70,6 +70,12 @@ class State:
         check(_ffi.lib().ev_sv_read(self._h, _ffi.ptr(out), out.size))
         return out
 
+    def amplitudes_range(self, offset: int, length: int) -> np.ndarray:
+        """Copies amplitudes [offset, offset + length) to the host."""
+        out = np.empty(length, dtype=np.complex128)
+        check(_ffi.lib().ev_sv_read_range(self._h, offset, _ffi.ptr(out), length))
+        return out
+
     def write(self, amps) -> None:
         a = np.ascontiguousarray(amps, dtype=np.complex128)
         check(_ffi.lib().ev_sv_write(self._h, _ffi.ptr(a), a.size))

@@ -126,6 +126,8 @@ uint64_t ev_sv_num_qubits(const ev_sv *s);                              /* state
 ev_precision ev_sv_precision(const ev_sv *s);
 int ev_sv_read(const ev_sv *s, ev_c128 *host, size_t len);              /* state.rs:81-83 amplitudes() (c64 widened) */
 int ev_sv_write(ev_sv *s, const ev_c128 *host, size_t len);
+/* Copies amplitudes [offset, offset + len) to the host (c64 widened). */
+int ev_sv_read_range(const ev_sv *s, uint64_t offset, ev_c128 *host, size_t len);
 int ev_sv_copy(const ev_sv *src, ev_sv **out);                          /* State: Clone */
 
 /* Per-gate device kernels (src/kernel.rs:76, :192, :299).  Preconditions are
@@ -227,6 +229,20 @@ int ev_dm_partial_trace_1q(const ev_dm *d, uint64_t qubit, ev_c128 out[4]);     
  * NULL (NoiseModel::ideal, density_matrix.rs:76-78). */
 int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, const ev_run_options *opt,
               ev_dm **out, uint8_t *classical, size_t num_classical, ev_error *err);
+
+/* ---------------------------------------------------------- introspection */
+
+/* The fused pass plan ev_sv_run would execute for a measurement-free circuit
+ * (descriptor layouts in everett_amd/csrc/fused.hpp).  Call once with NULL
+ * arrays to get the counts, then with arrays of at least those sizes.  coef
+ * is in c128 (a c64 run converts it to f32 on upload). */
+typedef struct {
+    uint32_t num_passes, num_stages, num_ops, num_coef;
+    uint32_t pass_bytes, stage_bytes, op_bytes;
+    uint32_t tile_qubits;
+} ev_plan_info;
+int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, void *stages, void *ops,
+                  ev_c128 *coef, ev_plan_info *info);
 
 /* ------------------------------------------------------------- profiling */
 

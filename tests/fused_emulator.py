@@ -1,0 +1,193 @@
+"""CPU emulation of the fused pass kernel (everett_amd/csrc/fused_kernels.hip).
+
+Replays a plan from ev_debug_plan with the kernel's exact index arithmetic
+(tile base, load mapping, XOR-swizzled LDS slots, per-stage register
+mapping, conditions) on a numpy state, asserting that every HBM index is in
+range and every LDS mapping is a permutation, so the plan and the kernel's
+indexing are checked on the CPU before a GPU runs them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from everett_amd import _ffi
+from everett_amd.errors import check
+
+PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"), ("ncomp", "<i4"),
+                       ("q", "i1", (16,)), ("comp", "i1", (40,))])
+STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("pad", "<i4"),
+                        ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "i1", (13,)), ("gpos", "i1", (13,)),
+                        ("pad2", "i1", (6,))])
+OP_DTYPE = np.dtype([("kind", "u1"), ("b0", "u1"), ("b1", "u1"), ("flags", "u1"), ("creg", "<u4"),
+                     ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4")])
+
+U1, CU1, CX, DIAGR, DIAGT, U2, SWAP = range(7)
+
+
+def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0):
+    from everett_amd.backend import run_options
+
+    opt = run_options(precision, False, tile_qubits, max_gates_per_pass)
+    info = _ffi.EvPlanInfo()
+    check(_ffi.lib().ev_debug_plan(circuit.handle, C.byref(opt), None, None, None, None, C.byref(info)))
+    assert info.pass_bytes == PASS_DTYPE.itemsize
+    assert info.stage_bytes == STAGE_DTYPE.itemsize
+    assert info.op_bytes == OP_DTYPE.itemsize
+    passes = np.zeros(info.num_passes, dtype=PASS_DTYPE)
+    stages = np.zeros(info.num_stages, dtype=STAGE_DTYPE)
+    ops = np.zeros(max(info.num_ops, 1), dtype=OP_DTYPE)
+    coef = np.zeros(max(info.num_coef, 1), dtype=np.complex128)
+    check(_ffi.lib().ev_debug_plan(circuit.handle, C.byref(opt), _ffi.ptr(passes), _ffi.ptr(stages), _ffi.ptr(ops),
+                                   _ffi.ptr(coef), C.byref(info)))
+    return passes, stages, ops[: info.num_ops], coef[: info.num_coef], info.tile_qubits
+
+
+def swz(l):
+    return l ^ (((l >> 4) ^ (l >> 8) ^ (l >> 12)) & 15)
+
+
+def _coef_span(kind):
+    return {U1: 4, CU1: 4, DIAGR: 2, DIAGT: 2, U2: 16}.get(kind, 0)
+
+
+def check_plan(n, passes, stages, ops, coef):
+    """Structural bounds of a plan, independent of any state."""
+    for P in passes:
+        t = int(P["t"])
+        assert 8 <= t <= 13 and t <= n and P["ncomp"] == n - t
+        qs = sorted(list(P["q"][:t]) + list(P["comp"][: n - t]))
+        assert qs == list(range(n)), "tile and complement must partition the qubits"
+        assert list(P["q"][:t]) == sorted(P["q"][:t])
+        assert P["nstages"] >= 1 and P["stage_begin"] + P["nstages"] <= len(stages)
+        W = 1 << (t - 4)
+        tid = np.arange(W)
+        for s in range(P["nstages"]):
+            st = stages[P["stage_begin"] + s]
+            assert st["np"] == t - 4
+            assert 0 <= st["op_begin"] <= st["op_end"] <= len(ops)
+            lb = np.zeros(W, dtype=np.int64)
+            for i in range(t - 4):
+                lb |= ((tid >> i) & 1) << int(st["p"][i])
+                assert P["q"][st["p"][i]] == st["gpos"][i]
+            slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+            assert slots.min() >= 0 and slots.max() < (1 << t)
+            assert len(np.unique(slots)) == (1 << t), "stage mapping must be a permutation of the tile"
+            regq = set()
+            for j in range(4):
+                gj = int(st["goff"][1 << j])
+                assert gj and gj & (gj - 1) == 0
+                regq.add(gj.bit_length() - 1)
+            for o in ops[st["op_begin"]: st["op_end"]]:
+                k = int(o["kind"])
+                assert k <= SWAP and o["b0"] < 4 and o["b1"] < 4 and o["creg"] < 16
+                if k in (U2, SWAP):
+                    assert o["b0"] > o["b1"]
+                assert o["coef"] + _coef_span(k) <= len(coef)
+                cg = int(o["cglob"])
+                assert cg < (1 << n) and all(not (cg >> q) & 1 for q in regq)
+                if k == DIAGT:
+                    assert o["tq"] < n and o["tq"] not in regq
+
+
+def _apply(o, v, gfix, coef):
+    k, B, BL = int(o["kind"]), int(o["b0"]), int(o["b1"])
+    creg, cglob = int(o["creg"]), np.uint64(o["cglob"])
+    tok = (gfix & cglob) == cglob
+    m = coef[int(o["coef"]):]
+    condj = [((j & creg) == creg) for j in range(16)]
+    if k in (U1, CU1):
+        for j in range(16):
+            if j & (1 << B):
+                continue
+            j1 = j | (1 << B)
+            rows = tok if k == CU1 else np.ones_like(tok)
+            if k == CU1 and not condj[j]:
+                continue
+            a0, a1 = v[:, j].copy(), v[:, j1].copy()
+            v[:, j] = np.where(rows, m[0] * a0 + m[1] * a1, a0)
+            v[:, j1] = np.where(rows, m[2] * a0 + m[3] * a1, a1)
+    elif k == CX:
+        for j in range(16):
+            if j & (1 << B) or not condj[j]:
+                continue
+            j1 = j | (1 << B)
+            a0, a1 = v[:, j].copy(), v[:, j1].copy()
+            v[:, j] = np.where(tok, a1, a0)
+            v[:, j1] = np.where(tok, a0, a1)
+    elif k == DIAGR:
+        one = bool(o["flags"] & 1)
+        for j in range(16):
+            if not condj[j]:
+                continue
+            if j & (1 << B):
+                v[:, j] = np.where(tok, m[1] * v[:, j], v[:, j])
+            elif not one:
+                v[:, j] = np.where(tok, m[0] * v[:, j], v[:, j])
+    elif k == DIAGT:
+        d = np.where(((gfix >> np.uint64(o["tq"])) & np.uint64(1)) == 1, m[1], m[0])
+        for j in range(16):
+            if condj[j]:
+                v[:, j] = np.where(tok, d * v[:, j], v[:, j])
+    elif k == U2:
+        M = m[:16].reshape(4, 4)
+        for j in range(16):
+            if j & ((1 << B) | (1 << BL)):
+                continue
+            idx = [j, j | (1 << BL), j | (1 << B), j | (1 << B) | (1 << BL)]
+            a = v[:, idx].copy()
+            v[:, idx] = a @ M.T
+    else:
+        for j in range(16):
+            if j & ((1 << B) | (1 << BL)):
+                continue
+            a, b = j | (1 << BL), j | (1 << B)
+            t = v[:, a].copy()
+            v[:, a] = v[:, b]
+            v[:, b] = t
+
+
+def emulate(psi, n, passes, stages, ops, coef):
+    """Runs the plan on psi (complex128, length 2^n) in place."""
+    for P in passes:
+        t = int(P["t"])
+        W = 1 << (t - 4)
+        tid = np.arange(W, dtype=np.int64)
+        q = [int(x) for x in P["q"][:t]]
+        comp = [int(x) for x in P["comp"][: n - t]]
+        gthr = np.zeros(W, dtype=np.int64)
+        for i in range(t - 4):
+            gthr |= ((tid >> i) & 1) << q[i]
+        for T in range(1 << (n - t)):
+            base = 0
+            for i, c in enumerate(comp):
+                base |= ((T >> i) & 1) << c
+            lds = np.full(1 << t, np.nan, dtype=np.complex128)
+            idxs = []
+            for j in range(16):
+                gj = 0
+                for kk in range(4):
+                    if (j >> kk) & 1:
+                        gj |= 1 << q[t - 4 + kk]
+                idx = base | gthr | gj
+                assert idx.max() < (1 << n)
+                lds[swz(j * W + tid)] = psi[idx]
+                idxs.append(idx)
+            assert not np.isnan(lds).any()
+            for s in range(P["nstages"]):
+                st = stages[P["stage_begin"] + s]
+                lb = np.zeros(W, dtype=np.int64)
+                gfix = np.full(W, base, dtype=np.uint64)
+                for i in range(t - 4):
+                    bit = ((tid >> i) & 1).astype(np.uint64)
+                    lb |= ((tid >> i) & 1) << int(st["p"][i])
+                    gfix |= bit << np.uint64(st["gpos"][i])
+                slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+                v = lds[slots]
+                for o in ops[st["op_begin"]: st["op_end"]]:
+                    _apply(o, v, gfix, coef)
+                lds[slots] = v
+            for j in range(16):
+                psi[idxs[j]] = lds[swz(j * W + tid)]
+    return psi

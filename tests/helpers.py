@@ -127,3 +127,19 @@ def ops_array(records):
             out[i]["num_controls"] = len(controls)
             out[i]["controls"][: len(controls)] = controls
     return out
+
+
+def adjoint_ops(ops: np.ndarray) -> np.ndarray:
+    """The inverse circuit of a measurement-free op list: reversed order, every matrix conjugate-transposed."""
+    out = ops[::-1].copy()
+    for rec in out:
+        k = int(rec["kind"])
+        if k in (1, 3):
+            m = rec["m"][:4].reshape(2, 2)
+            rec["m"][:4] = m.conj().T.reshape(4)
+        elif k == 2:
+            m = rec["m"].reshape(4, 4)
+            rec["m"][:] = m.conj().T.reshape(16)
+        else:
+            raise ValueError("adjoint_ops takes gate-only circuits")
+    return out

@@ -1,0 +1,140 @@
+"""Fused passes (planner + pass kernel) on the device against the oracle.
+
+The fused path multiplies consecutive one-qubit gates in f64 and evaluates
+gates in a different association order than the reference, so complex128
+results are held to 1e-10 max-abs and complex64 to
+1e-5 max-abs with 1 - fidelity <= 1e-6.  At the full 30-qubit size the checks
+are size-independent: a circuit followed by its inverse returns |0..0>, the
+norm is preserved, and QFT of a basis state matches its closed form.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import everett_amd as ev
+import oracle
+from tests.helpers import adjoint_ops, random_unitary
+from tests.test_planner import mixed_circuit
+
+pytestmark = pytest.mark.gpu
+
+TOL128 = 1e-10
+
+
+@pytest.mark.parametrize("n,depth,tile", [(8, 6, 0), (12, 10, 0), (13, 8, 9), (16, 12, 0)])
+def test_fused_brickwall_qft_matches_oracle(n, depth, tile):
+    c = ev.workloads.config1(n, depth)
+    want, _ = oracle.sv_run(c.ops(), n)
+    got = ev.StateVectorBackend.run(c, tile_qubits=tile).state().amplitudes()
+    assert np.max(np.abs(got - want)) < TOL128
+
+
+@pytest.mark.parametrize("n,seed,budget", [(9, 1, 0), (10, 2, 3), (12, 3, 0), (14, 4, 50)])
+def test_fused_mixed_circuits_match_oracle(n, seed, budget):
+    c = mixed_circuit(n, 300, seed)
+    want, _ = oracle.sv_run(c.ops(), n)
+    got = ev.StateVectorBackend.run(c, max_gates_per_pass=budget).state().amplitudes()
+    assert np.max(np.abs(got - want)) < TOL128
+
+
+def test_fused_config0_matches_oracle():
+    c = ev.workloads.config0()
+    want, _ = oracle.sv_run(c.ops(), 20)
+    got = ev.StateVectorBackend.run(c).state().amplitudes()
+    assert np.max(np.abs(got - want)) < TOL128
+
+
+def test_fused_with_measurement_segments():
+    n = 10
+    c = ev.Circuit.with_classical(n, 3)
+    c.compose(ev.workloads.config0(n, 3))
+    c.measure(3, 0).measure(7, 1)
+    c.x_if(0, 2).z_if(1, 5)
+    c.compose(ev.workloads.brickwall(n, 2, ev.Rng.seed_from_u64(5)))
+    c.measure(0, 2)
+    for seed in (0, 1, 2, 3):
+        ex = ev.StateVectorBackend.run_seeded(c, seed)
+        want, bits = oracle.sv_run(c.ops(), n, seed, num_classical=3)
+        assert ex.classical() == bits
+        assert np.max(np.abs(ex.state().amplitudes() - want)) < TOL128
+
+
+@pytest.mark.parametrize("n", [12, 18])
+def test_fused_c64_fidelity(n):
+    c = ev.workloads.config1(n, 20)
+    want, _ = oracle.sv_run(c.ops(), n)
+    got = ev.StateVectorBackend.run(c, precision="c64").state().amplitudes()
+    fid = abs(np.vdot(want, got)) ** 2
+    assert np.max(np.abs(got - want)) < 1e-5 and 1 - fid < 1e-6
+
+
+@pytest.mark.parametrize("n", [24, 28])
+def test_fused_matches_plain_at_scale(n):
+    # the plain path equals the oracle exactly (test_gpu_backends), so it is the reference here
+    c = ev.workloads.config1(n, 40)
+    a = ev.StateVectorBackend.run(c).state()
+    b = ev.StateVectorBackend.run(c, plain=True).state()
+    off = (1 << n) // 3
+    for lo in (0, off, (1 << n) - 65536):
+        x = a.amplitudes_range(lo, 65536)
+        y = b.amplitudes_range(lo, 65536)
+        assert np.max(np.abs(x - y)) < TOL128
+    assert abs(a.fidelity(b) - 1.0) < 1e-10
+
+
+def test_config1_round_trip_at_30_qubits():
+    c = ev.workloads.config1(30, 40)
+    inv = ev.Circuit(30)
+    inv.append_ops(adjoint_ops(c.ops()))
+    c.compose(inv)
+    s = ev.StateVectorBackend.run(c).state()
+    head = s.amplitudes_range(0, 4096)
+    assert abs(head[0] - 1.0) < 1e-9
+    assert np.max(np.abs(head[1:])) < 1e-9
+    assert abs(s.norm_sqr() - 1.0) < 1e-10
+
+
+def test_qft_basis_state_closed_form_at_30_qubits():
+    n, j = 30, 0x2B5A3C1
+    c = ev.Circuit(n)
+    for q in range(n):
+        if (j >> q) & 1:
+            c.x(q)
+    c.compose(ev.algorithms.qft(n))
+    s = ev.StateVectorBackend.run(c).state()
+    N = 1 << n
+    for lo in (0, 123456789, N - 4096):
+        k = np.arange(lo, lo + 4096, dtype=np.float64)
+        want = np.exp(2j * np.pi * ((j * k) % N) / N) / math.sqrt(N)
+        got = s.amplitudes_range(lo, 4096)
+        assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("name,noise,ref", [
+    ("ideal", ev.NoiseModel.ideal(), {}),
+    ("depol", ev.NoiseModel.uniform_depolarizing(0.05),
+     {"kind_1q": "depolarizing", "p_1q": 0.05, "kind_2q": "depolarizing", "p_2q": 0.05}),
+    ("ad", ev.NoiseModel.amplitude_damping(0.1),
+     {"kind_1q": "amplitude_damping", "p_1q": 0.1, "kind_2q": "amplitude_damping", "p_2q": 0.1}),
+])
+def test_fused_dm_matches_oracle(name, noise, ref):
+    from tests.test_gpu_backends import random_circuit
+
+    for n, seed in ((4, 1), (5, 2)):
+        c = random_circuit(n, 40, seed, ncl=2)
+        for s in (0, 4):
+            ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, s)
+            want, bits = oracle.dm_run(c.ops(), n, ref, s, num_classical=2)
+            assert ex.classical() == bits, (name, n, s)
+            assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10
+
+
+def test_fused_config2_reduced_matches_oracle():
+    c, noise = ev.workloads.config2(6, 8)
+    ks = np.array([k.m for k in noise.after_1q.ops])
+    ref = {"custom_1q": ks, "custom_2q": ks, "readout_error": 0.01}
+    ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 7)
+    want, bits = oracle.dm_run(c.ops(), 6, ref, 7, num_classical=6)
+    assert ex.classical() == bits
+    assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10

@@ -1,0 +1,98 @@
+"""Fused pass planner on the CPU: structural bounds of every plan and an
+emulation of the pass kernel's indexing against the oracle (no GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+import everett_amd as ev
+import oracle
+from tests.fused_emulator import check_plan, emulate, get_plan
+from tests.helpers import random_unitary
+
+
+def mixed_circuit(n, nops, seed):
+    rng = np.random.default_rng(seed)
+    c = ev.Circuit(n)
+    for _ in range(nops):
+        r = int(rng.integers(0, 12))
+        q = int(rng.integers(0, n))
+        b = int(rng.integers(0, n - 1))
+        b = b + 1 if b >= q else b
+        if r < 3:
+            c.gate1(ev.Gate1(*random_unitary(rng, 2).reshape(4)), q)
+        elif r == 3:
+            c.rz(q, float(rng.uniform(0, 6.28)))
+        elif r == 4:
+            c.cnot(q, b)
+        elif r == 5:
+            c.cz(q, b)
+        elif r == 6:
+            c.swap(q, b)
+        elif r == 7:
+            c.gate2(ev.Gate2(random_unitary(rng, 4)), q, b)
+        elif r == 8:
+            others = [x for x in range(n) if x != q]
+            cs = [int(x) for x in rng.choice(others, size=int(rng.integers(1, 4)), replace=False)]
+            c.controlled(cs, ev.Gate1(*random_unitary(rng, 2).reshape(4)), q)
+        elif r == 9:
+            c.controlled([b], ev.Gate1.phase(float(rng.uniform(0, 6.28))), q)
+        elif r == 10:
+            others = [x for x in range(n) if x != q]
+            cs = [int(x) for x in rng.choice(others, size=2, replace=False)]
+            c.controlled(cs, ev.Gate1.x(), q)
+        else:
+            c.h(q)
+    return c
+
+
+CASES = [
+    ("brick10+qft", lambda: ev.workloads.config1(10, 6), {}),
+    ("brick12", lambda: ev.workloads.config0(12, 8), {"tile_qubits": 9}),
+    ("brick11-budget1", lambda: ev.workloads.config0(11, 4), {"max_gates_per_pass": 1, "tile_qubits": 8}),
+    ("brick11-budget-big", lambda: ev.workloads.config0(11, 6), {"max_gates_per_pass": 1000, "tile_qubits": 8}),
+    ("mixed10", lambda: mixed_circuit(10, 120, 1), {"tile_qubits": 8}),
+    ("mixed11", lambda: mixed_circuit(11, 200, 2), {"tile_qubits": 10}),
+    ("mixed9-c64", lambda: mixed_circuit(9, 150, 3), {"precision": "c64", "tile_qubits": 9}),
+    ("qft12", lambda: ev.algorithms.qft(12), {"tile_qubits": 10}),
+    ("mixed13", lambda: mixed_circuit(13, 160, 4), {"tile_qubits": 12}),
+]
+
+
+@pytest.mark.parametrize("name,make,opts", CASES, ids=[c[0] for c in CASES])
+def test_plan_emulation_matches_oracle(name, make, opts):
+    c = make()
+    n = c.num_qubits()
+    passes, stages, ops, coef, t = get_plan(c, **opts)
+    check_plan(n, passes, stages, ops, coef)
+    rng = np.random.default_rng(n)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    want = psi.copy()
+    oracle.sv_apply_ops(c.ops(), want)
+    got = emulate(psi.copy(), n, passes, stages, ops, coef)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_plan_structure_at_30_qubits():
+    c = ev.workloads.config1()
+    passes, stages, ops, coef, t = get_plan(c)
+    check_plan(30, passes, stages, ops, coef)
+    assert t == 12
+    # fusion must beat one sweep per gate by a wide margin
+    assert len(passes) < len(c) / 20
+
+
+def test_plan_structure_c64_30_qubits():
+    c = ev.workloads.config1()
+    passes, stages, ops, coef, t = get_plan(c, precision="c64")
+    check_plan(30, passes, stages, ops, coef)
+    assert t == 13
+
+
+def test_tile_option_bounds():
+    c = ev.workloads.config0(12, 2)
+    with pytest.raises(ev.InvalidArgument):
+        get_plan(c, tile_qubits=13)  # 2^13 c128 amplitudes exceed 64 KiB of LDS
+    with pytest.raises(ev.InvalidArgument):
+        get_plan(c, tile_qubits=7)
