@@ -27,7 +27,8 @@ namespace ev::fused {
 
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;  // amplitudes per thread per stage
-constexpr int kMaxTile = 13;          // 2^13 x 8 B (c64) or 2^12 x 16 B (c128) = 64 KiB of LDS
+constexpr int kMaxTile = 12;          // 2^12 amplitudes: 256 threads x 16 (64 KiB of LDS at c128, 32 KiB at c64)
+constexpr int kMaxThreads = 1 << (kMaxTile - kRegBits);
 constexpr int kMinTile = 8;
 
 enum OpKind : uint8_t {
@@ -56,8 +57,8 @@ struct Stage {
     int32_t pad;
     uint32_t roff[kRegs];     // swizzled LDS slot offset of register j
     uint64_t goff[kRegs];     // global index offset of register j
-    int8_t p[kMaxTile];       // thread-id bit i -> local position p[i]
-    int8_t gpos[kMaxTile];    // thread-id bit i -> global qubit
+    int8_t p[13];             // thread-id bit i -> local position p[i]
+    int8_t gpos[13];          // thread-id bit i -> global qubit
     int8_t pad2[6];
 };
 

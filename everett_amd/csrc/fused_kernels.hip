@@ -11,6 +11,7 @@
 // reference's per-product rounding to ~1e-15 relative per gate.
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
 #include "device.hpp"
 #include "fused.hpp"
 #include "fused_kernels.hpp"
@@ -182,7 +183,7 @@ __device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfi
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_fused_pass(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
+__global__ __launch_bounds__(kMaxThreads) void k_fused_pass(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
                                                     const Stage *__restrict__ stages, const DevOp *__restrict__ ops,
                                                     const Amp<R> *__restrict__ coef)
 {
@@ -248,8 +249,9 @@ __global__ __launch_bounds__(256) void k_fused_pass(Amp<R> *__restrict__ psi, co
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
                  const void *coef, hipStream_t st)
 {
+    EV_REQUIRE(t >= kMinTile && t <= kMaxTile && t <= n, EV_E_ARG, "fused pass tile width out of range");
     const dim3 grid(static_cast<unsigned>(uint64_t(1) << (n - t)));
-    const dim3 block(1u << (t - kRegBits));
+    const dim3 block(1u << (t - kRegBits));  // <= kMaxThreads = the kernel's launch bound
     if (!c64) {
         const size_t lds = (size_t(1) << t) * sizeof(double2);
         hipLaunchKernelGGL(k_fused_pass<double>, grid, block, lds, st, static_cast<double2 *>(psi), pass, stages, ops,

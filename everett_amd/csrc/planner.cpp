@@ -376,7 +376,7 @@ struct FusedConfig {
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 {
     FusedConfig f;
-    const int tmax = c64 ? fused::kMaxTile : fused::kMaxTile - 1;  // 64 KiB of LDS per tile
+    const int tmax = fused::kMaxTile;  // 256 threads x 16 amplitudes
     f.t = opt.tile_qubits ? int(opt.tile_qubits) : tmax;
     EV_REQUIRE(f.t >= fused::kMinTile && f.t <= tmax, EV_E_ARG,
                "tile_qubits must be in [8, " + std::to_string(tmax) + "] for this precision");

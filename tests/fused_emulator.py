@@ -56,7 +56,7 @@ def check_plan(n, passes, stages, ops, coef):
     """Structural bounds of a plan, independent of any state."""
     for P in passes:
         t = int(P["t"])
-        assert 8 <= t <= 13 and t <= n and P["ncomp"] == n - t
+        assert 8 <= t <= 12 and t <= n and P["ncomp"] == n - t
         qs = sorted(list(P["q"][:t]) + list(P["comp"][: n - t]))
         assert qs == list(range(n)), "tile and complement must partition the qubits"
         assert list(P["q"][:t]) == sorted(P["q"][:t])
