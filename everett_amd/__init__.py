@@ -7,6 +7,8 @@ reference's public names (Circuit, Gate1/Gate2, StateVectorBackend,
 DensityMatrixBackend, NoiseModel, Channel, KrausOp, Rng, State).
 """
 from . import algorithms, kernel, workloads
+from .batch import Batch
+from .server import Server
 from .backend import (
     DensityMatrix,
     DensityMatrixBackend,
@@ -38,7 +40,7 @@ from .rng import Rng
 from .state import State, measure_qubit
 
 __all__ = [
-    "algorithms", "kernel", "workloads", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
+    "algorithms", "kernel", "workloads", "Batch", "Server", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
     "StateVectorBackend", "Execution", "DensityMatrixBackend", "DensityMatrix", "DensityMatrixExecution",
     "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "EverettError", "QubitOutOfRange",
     "ClassicalBitOutOfRange", "DuplicateQubit", "DimensionMismatch", "InvalidArgument", "DeviceError",

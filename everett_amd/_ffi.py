@@ -62,7 +62,9 @@ class EvRunOptions(C.Structure):
         ("tile_qubits", C.c_uint32),
         ("max_gates_per_pass", C.c_uint32),
         ("profile", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("use_graph", C.c_uint32),
+        ("renormalize", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -88,6 +90,31 @@ class EvRunStats(C.Structure):
 
 class EvKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
+class EvBatchStats(C.Structure):
+    _fields_ = [
+        ("circuits", C.c_uint64), ("logical_gates", C.c_uint64), ("rounds", C.c_uint64), ("passes", C.c_uint64),
+        ("tiles", C.c_uint64), ("plan_ms", C.c_double), ("upload_ms", C.c_double), ("device_ms", C.c_double),
+        ("renorm_ms", C.c_double), ("graph", C.c_uint32), ("pad", C.c_uint32),
+    ]
+
+
+class EvServerConfig(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_active", C.c_uint32), ("memory_bytes", C.c_uint64),
+                ("tile_qubits", C.c_uint32), ("max_gates_per_pass", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class EvRequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("num_qubits", C.c_uint64), ("gates", C.c_uint64), ("passes", C.c_uint64),
+                ("shots", C.c_uint64), ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+                ("status", C.c_int32), ("pad", C.c_int32)]
+
+
+class EvServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
+                ("active", C.c_uint64), ("queued", C.c_uint64), ("bytes_in_use", C.c_uint64),
+                ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double)]
 
 
 class EvPlanInfo(C.Structure):
@@ -164,6 +191,20 @@ PROTOTYPES = {
     "ev_last_run_stats": (I, [C.POINTER(EvRunStats)]),
     "ev_last_run_kernel_stats": (I, [C.POINTER(EvKernelStat), SZ, C.POINTER(SZ)]),
     "ev_debug_plan": (I, [P, C.POINTER(EvRunOptions), P, P, P, P, C.POINTER(EvPlanInfo)]),
+    "ev_batch_run": (I, [P, SZ, C.POINTER(EvRunOptions), PP, C.POINTER(EvError)]),
+    "ev_batch_destroy": (None, [P]),
+    "ev_batch_count": (SZ, [P]),
+    "ev_batch_num_qubits": (U64, [P, SZ]),
+    "ev_batch_read": (I, [P, SZ, U64, P, SZ]),
+    "ev_batch_norms": (I, [P, P]),
+    "ev_batch_compare": (I, [P, P, P, P]),
+    "ev_batch_get_stats": (I, [P, C.POINTER(EvBatchStats)]),
+    "ev_server_create": (I, [C.POINTER(EvServerConfig), PP]),
+    "ev_server_destroy": (None, [P]),
+    "ev_server_submit": (I, [P, P, U64, U64, C.POINTER(U64)]),
+    "ev_server_poll": (I, [P, U64, C.POINTER(I)]),
+    "ev_server_wait": (I, [P, U64, P, C.POINTER(EvRequestInfo)]),
+    "ev_server_get_stats": (I, [P, C.POINTER(EvServerStats)]),
 }
 
 _lib = None

@@ -1,0 +1,324 @@
+// Batched execution of many independent statevector circuits.
+//
+// All states live in one HBM arena, packed back to back.  Each circuit is
+// planned into fused passes (in parallel on host threads); round r is ONE
+// persistent launch whose workgroups stride over every tile of the r-th pass
+// of every circuit that has one, so small and large states share the GPU.
+// The rounds can be captured into a HIP graph and launched once.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "batch.hpp"
+#include "batch_kernels.hpp"
+#include "fused_kernels.hpp"
+#include "planner.hpp"
+
+namespace ev {
+
+namespace {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Device layout of one circuit's plan inside the descriptor buffer.
+struct PlanLoc {
+    size_t pass_off, stage_off, op_off, coef_off;
+    uint32_t npass;
+};
+
+}  // namespace
+
+void batch_free(ev_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->arena) (void)hipFree(b->arena);
+    if (b->desc) (void)hipFree(b->desc);
+    if (b->d_meta) (void)hipFree(b->d_meta);
+    if (b->d_part) (void)hipFree(b->d_part);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options &opt)
+{
+    EV_REQUIRE(count > 0, EV_E_ARG, "empty batch");
+    EV_REQUIRE(count < (1u << 24), EV_E_LIMIT, "batch too large");
+    const bool c64 = opt.precision == EV_C64;
+    const size_t ab = c64 ? 8 : 16;
+    ev_batch *b = new ev_batch;
+    b->prec = c64 ? kern::C64 : kern::C128;
+    try {
+        // ---- validate and size
+        b->n.resize(count);
+        b->aoff.resize(count + 1);
+        b->gates.resize(count);
+        uint64_t amps = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const Circuit &c = *cs[i];
+            c.validate();
+            const uint64_t n = c.num_qubits();
+            EV_REQUIRE(n >= uint64_t(fused::kMinTile) && n <= 30, EV_E_LIMIT, "batched circuits need 8..30 qubits");
+            for (const Op &op : c.ops())
+                EV_REQUIRE(op.kind == OpKind::Apply1 || op.kind == OpKind::Apply2 || op.kind == OpKind::Controlled,
+                           EV_E_ARG, "batched circuits must be measurement-free");
+            b->n[i] = int(n);
+            b->gates[i] = c.size();
+            b->aoff[i] = amps;
+            amps += uint64_t(1) << n;  // powers of two >= 256, so every offset stays a multiple of kChunk
+        }
+        b->aoff[count] = amps;
+        b->total_amps = amps;
+
+        // ---- plan every circuit on host threads
+        const double t_plan = now_ms();
+        std::vector<Plan> plans(count);
+        {
+            std::atomic<size_t> next{0};
+            const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            std::vector<std::exception_ptr> errs(nt);
+            for (unsigned w = 0; w < nt; ++w)
+                pool.emplace_back([&, w] {
+                    try {
+                        for (size_t i; (i = next.fetch_add(1)) < count;) {
+                            const FusedConfig cfg = fused_config(b->n[i], c64, opt);
+                            plans[i] = plan_circuit(*cs[i], b->n[i], cfg);
+                        }
+                    } catch (...) {
+                        errs[w] = std::current_exception();
+                    }
+                });
+            for (auto &th : pool) th.join();
+            for (auto &e : errs)
+                if (e) std::rethrow_exception(e);
+        }
+        b->stats.plan_ms = now_ms() - t_plan;
+
+        // ---- descriptor layout: per-circuit plans, then the round tables
+        std::vector<PlanLoc> loc(count);
+        size_t off = 0;
+        uint32_t max_passes = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const Plan &P = plans[i];
+            loc[i].pass_off = off;
+            off += align256(P.passes.size() * sizeof(fused::Pass));
+            loc[i].stage_off = off;
+            off += align256(P.stages.size() * sizeof(fused::Stage));
+            loc[i].op_off = off;
+            off += align256(std::max<size_t>(1, P.devops.size()) * sizeof(fused::DevOp));
+            loc[i].coef_off = off;
+            off += align256(std::max<size_t>(1, P.coef.size()) * ab);
+            loc[i].npass = uint32_t(P.passes.size());
+            max_passes = std::max(max_passes, loc[i].npass);
+            b->stats.passes += P.passes.size();
+        }
+        const size_t entries_off = off;
+        std::vector<size_t> round_off(max_passes), round_n(max_passes);
+        std::vector<uint64_t> round_tiles(max_passes);
+        std::vector<int> round_t(max_passes, fused::kMinTile);
+        size_t nent = 0;
+        for (uint32_t r = 0; r < max_passes; ++r) {
+            round_off[r] = nent;
+            for (size_t i = 0; i < count; ++i)
+                if (loc[i].npass > r) ++nent;
+            round_n[r] = nent - round_off[r];
+        }
+        off += align256(std::max<size_t>(1, nent) * sizeof(fused::BatchEntry));
+        const size_t desc_bytes = off;
+
+        // ---- device memory
+        EV_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        EV_HIP(hipMalloc(&b->arena, size_t(amps) * ab));
+        EV_HIP(hipMalloc(&b->desc, desc_bytes));
+        EV_HIP(hipMalloc(&b->d_meta, sizeof(uint64_t) * (2 * (count + 1))));
+        const uint64_t nchunks = amps / batchk::kChunk;
+        EV_HIP(hipMalloc(&b->d_part, sizeof(double) * (5 * nchunks + 5 * count + count)));
+        EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->h_out), sizeof(double) * 5 * count, hipHostMallocDefault));
+
+        // ---- host image of the descriptors, with device pointers in the entries
+        const double t_up = now_ms();
+        std::vector<unsigned char> host(desc_bytes, 0);
+        unsigned char *dbase = static_cast<unsigned char *>(b->desc);
+        char *abase = static_cast<char *>(b->arena);
+        for (size_t i = 0; i < count; ++i) {
+            const Plan &P = plans[i];
+            std::memcpy(host.data() + loc[i].pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
+            std::memcpy(host.data() + loc[i].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
+            if (!P.devops.empty())
+                std::memcpy(host.data() + loc[i].op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
+            if (!c64) {
+                std::memcpy(host.data() + loc[i].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
+            } else {
+                float *f = reinterpret_cast<float *>(host.data() + loc[i].coef_off);
+                for (size_t k = 0; k < P.coef.size(); ++k) {
+                    f[2 * k] = float(P.coef[k].re);
+                    f[2 * k + 1] = float(P.coef[k].im);
+                }
+            }
+        }
+        fused::BatchEntry *ent = reinterpret_cast<fused::BatchEntry *>(host.data() + entries_off);
+        for (uint32_t r = 0; r < max_passes; ++r) {
+            uint64_t tiles = 0;
+            size_t k = round_off[r];
+            for (size_t i = 0; i < count; ++i) {
+                if (loc[i].npass <= r) continue;
+                const fused::Pass &ps = plans[i].passes[r];
+                fused::BatchEntry &e = ent[k++];
+                e.psi = abase + b->aoff[i] * ab;
+                e.pass = reinterpret_cast<const fused::Pass *>(dbase + loc[i].pass_off) + r;
+                e.stages = reinterpret_cast<const fused::Stage *>(dbase + loc[i].stage_off);
+                e.ops = reinterpret_cast<const fused::DevOp *>(dbase + loc[i].op_off);
+                e.coef = dbase + loc[i].coef_off;
+                e.tile_begin = tiles;
+                tiles += uint64_t(1) << (b->n[i] - ps.t);
+                round_t[r] = std::max(round_t[r], int(ps.t));
+            }
+            round_tiles[r] = tiles;
+            b->stats.tiles += tiles;
+        }
+        plans.clear();
+        plans.shrink_to_fit();
+        std::vector<uint64_t> meta(2 * (count + 1));
+        for (size_t i = 0; i <= count; ++i) {
+            meta[i] = b->aoff[i];
+            meta[count + 1 + i] = b->aoff[i] / batchk::kChunk;
+        }
+        EV_HIP(hipMemcpyAsync(b->desc, host.data(), desc_bytes, hipMemcpyHostToDevice, b->stream));
+        EV_HIP(hipMemcpyAsync(b->d_meta, meta.data(), meta.size() * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+        batchk::init(b->arena, c64, amps, b->d_meta, int(count), b->stream);
+        EV_HIP(hipStreamSynchronize(b->stream));
+        b->stats.upload_ms = now_ms() - t_up;
+
+        // ---- rounds (optionally as one captured graph)
+        const fused::BatchEntry *dent = reinterpret_cast<const fused::BatchEntry *>(dbase + entries_off);
+        auto enqueue = [&] {
+            for (uint32_t r = 0; r < max_passes; ++r) {
+                const int t = round_t[r];
+                const unsigned per_cu = unsigned(std::max<size_t>(1, (160u << 10) / ((size_t(1) << t) * ab)));
+                const unsigned grid = 256u * std::min(per_cu, 8u);
+                fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], t, grid, b->stream);
+            }
+        };
+        hipEvent_t e0, e1;
+        EV_HIP(hipEventCreate(&e0));
+        EV_HIP(hipEventCreate(&e1));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        auto cleanup_graph = [&] {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        };
+        try {
+            if (opt.use_graph) {
+                EV_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                enqueue();
+                EV_HIP(hipStreamEndCapture(b->stream, &graph));
+                EV_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                b->stats.graph = 1;
+            }
+            EV_HIP(hipEventRecord(e0, b->stream));
+            if (gexec) EV_HIP(hipGraphLaunch(gexec, b->stream));
+            else enqueue();
+            EV_HIP(hipEventRecord(e1, b->stream));
+            EV_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            EV_HIP(hipEventElapsedTime(&ms, e0, e1));
+            b->stats.device_ms = ms;
+        } catch (...) {
+            cleanup_graph();
+            throw;
+        }
+        cleanup_graph();
+        b->stats.rounds = max_passes;
+        b->stats.circuits = count;
+        for (auto g : b->gates) b->stats.logical_gates += g;
+
+        // ---- f64 renormalisation of every state (c64 drift over deep circuits)
+        if (opt.renormalize) {
+            const double t_rn = now_ms();
+            std::vector<double> nrm(count);
+            batch_norms(b, nrm.data());
+            for (auto &x : nrm) x = x > 0 ? 1.0 / std::sqrt(x) : 0.0;
+            double *d_scale = b->d_part + 5 * nchunks + 5 * count;
+            EV_HIP(hipMemcpyAsync(d_scale, nrm.data(), count * sizeof(double), hipMemcpyHostToDevice, b->stream));
+            batchk::scale(b->arena, c64, amps, b->d_meta + count + 1, int(count), d_scale, b->stream);
+            EV_HIP(hipStreamSynchronize(b->stream));
+            b->stats.renorm_ms = now_ms() - t_rn;
+        }
+    } catch (...) {
+        batch_free(b);
+        throw;
+    }
+    return b;
+}
+
+void batch_norms(ev_batch *b, double *out)
+{
+    const size_t count = b->n.size();
+    const uint64_t nchunks = b->total_amps / batchk::kChunk;
+    double *res = b->d_part + 5 * nchunks;
+    batchk::norms(b->arena, b->prec == kern::C64, b->total_amps, b->d_meta + count + 1, int(count), b->d_part, res,
+                  b->stream);
+    EV_HIP(hipMemcpyAsync(b->h_out, res, count * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    EV_HIP(hipStreamSynchronize(b->stream));
+    std::memcpy(out, b->h_out, count * sizeof(double));
+}
+
+void batch_compare(ev_batch *b, ev_batch *ref, double *fid, double *maxabs)
+{
+    const size_t count = b->n.size();
+    EV_REQUIRE(ref->n == b->n, EV_E_ARG, "batches hold different circuit sizes");
+    EV_REQUIRE(ref->prec == kern::C128, EV_E_ARG, "the reference batch must be c128");
+    EV_HIP(hipStreamSynchronize(ref->stream));
+    const uint64_t nchunks = b->total_amps / batchk::kChunk;
+    double *res = b->d_part + 5 * nchunks;
+    batchk::compare(b->arena, b->prec == kern::C64, ref->arena, b->total_amps, b->d_meta + count + 1, int(count), b->d_part,
+                    res, b->stream);
+    EV_HIP(hipMemcpyAsync(b->h_out, res, 5 * count * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    EV_HIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < count; ++i) {
+        const double *r = b->h_out + 5 * i;
+        const double ov = r[0] * r[0] + r[1] * r[1];
+        fid[i] = (r[2] > 0 && r[3] > 0) ? ov / (r[2] * r[3]) : 0.0;
+        maxabs[i] = r[4];
+    }
+}
+
+void batch_read(ev_batch *b, size_t i, uint64_t offset, c128 *host, uint64_t len)
+{
+    EV_REQUIRE(i < b->n.size(), EV_E_ARG, "batch index out of range");
+    const uint64_t dim = uint64_t(1) << b->n[i];
+    EV_REQUIRE(offset <= dim && len <= dim - offset, EV_E_ARG, "amplitude range out of bounds");
+    if (len == 0) return;
+    const char *src = static_cast<const char *>(b->arena) + (b->aoff[i] + offset) * kern::amp_bytes(b->prec);
+    if (b->prec == kern::C128) {
+        EV_HIP(hipMemcpyAsync(host, src, len * 16, hipMemcpyDeviceToHost, b->stream));
+        EV_HIP(hipStreamSynchronize(b->stream));
+        return;
+    }
+    void *tmp = nullptr;
+    EV_HIP(hipMalloc(&tmp, len * sizeof(c128)));
+    kern::c64_to_c128(src, tmp, len, b->stream);
+    const hipError_t e = hipMemcpyAsync(host, tmp, len * sizeof(c128), hipMemcpyDeviceToHost, b->stream);
+    const hipError_t e2 = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    EV_HIP(e);
+    EV_HIP(e2);
+}
+
+}  // namespace ev

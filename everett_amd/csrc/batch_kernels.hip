@@ -1,0 +1,181 @@
+// Kernels over a batch arena: many statevectors packed back to back (each a
+// power-of-two run of amplitudes starting at a multiple of 256 amplitudes).
+// Reductions are chunked by 256 amplitudes (a chunk never straddles two
+// states) and combined per state in a fixed order, so results are
+// deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "batch_kernels.hpp"
+#include "device.hpp"
+
+namespace ev::batchk {
+
+namespace {
+
+constexpr int kBlock = 256;  // == kChunk
+
+#define GRID_STRIDE(i, count)                                                               \
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (count);         \
+         i += uint64_t(gridDim.x) * blockDim.x)
+
+inline dim3 grid_for(uint64_t work)
+{
+    uint64_t b = (work + kBlock - 1) / kBlock;
+    return dim3(static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 16))));
+}
+
+__device__ __forceinline__ double2 widen(double2 a) { return a; }
+__device__ __forceinline__ double2 widen(float2 a) { return make_double2(a.x, a.y); }
+
+template <class R> __global__ void k_zero(Amp<R> *a, uint64_t len)
+{
+    GRID_STRIDE(i, len) a[i] = mk<Amp<R>>(R(0), R(0));
+}
+
+template <class R> __global__ void k_set_first(Amp<R> *a, const uint64_t *aoff, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[aoff[i]] = mk<Amp<R>>(R(1), R(0));
+}
+
+// Block-wide sum of NV doubles; thread 0 gets the totals.
+template <int NV> __device__ void block_reduce(double (&v)[NV], double *out)
+{
+    __shared__ double sh[NV][kBlock];
+    for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < unsigned(s))
+            for (int c = 0; c < NV; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) out[c] = sh[c][0];
+    __syncthreads();
+}
+
+template <class R> __global__ void k_chunk_norm(const Amp<R> *a, uint64_t nchunks, double *part)
+{
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const double2 x = widen(a[c * kChunk + threadIdx.x]);
+        double v[1] = {__builtin_fma(x.x, x.x, x.y * x.y)};
+        double out[1];
+        block_reduce<1>(v, out);
+        if (threadIdx.x == 0) part[c] = out[0];
+    }
+}
+
+// Per chunk: <ref|x> (re, im), <x|x>, <ref|ref> and max |x - ref| (the max is
+// exact in any order).
+template <class R> __global__ void k_chunk_compare(const Amp<R> *x, const double2 *ref, uint64_t nchunks, double *part)
+{
+    __shared__ double mx[kBlock];
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const double2 a = widen(x[c * kChunk + threadIdx.x]);
+        const double2 r = ref[c * kChunk + threadIdx.x];
+        double v[4];
+        v[0] = __builtin_fma(r.x, a.x, r.y * a.y);   // re conj(r) * a
+        v[1] = __builtin_fma(r.x, a.y, -(r.y * a.x));  // im conj(r) * a
+        v[2] = __builtin_fma(a.x, a.x, a.y * a.y);
+        v[3] = __builtin_fma(r.x, r.x, r.y * r.y);
+        const double dx = a.x - r.x, dy = a.y - r.y;
+        mx[threadIdx.x] = __builtin_sqrt(dx * dx + dy * dy);
+        double out[4];
+        block_reduce<4>(v, out);
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < unsigned(s)) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 4; ++k) part[5 * c + k] = out[k];
+            part[5 * c + 4] = mx[0];
+        }
+        __syncthreads();
+    }
+}
+
+// One thread per state: sequential sum (or max for slot `max_slot`) over the
+// state's chunks.
+__global__ void k_combine(const double *part, int nv, int max_slot, const uint64_t *chunk_begin, int count, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    for (int k = 0; k < nv; ++k) {
+        double s = 0.0;
+        for (uint64_t c = chunk_begin[i]; c < chunk_begin[i + 1]; ++c)
+            s = (k == max_slot) ? fmax(s, part[nv * c + k]) : s + part[nv * c + k];
+        out[nv * i + k] = s;
+    }
+}
+
+template <class R>
+__global__ void k_scale_states(Amp<R> *a, uint64_t nchunks, const uint64_t *chunk_begin, int count, const double *scale)
+{
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        int lo = 0, hi = count - 1;  // last state with chunk_begin <= c
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (chunk_begin[mid] <= c) lo = mid; else hi = mid - 1;
+        }
+        const double s = scale[lo];
+        Amp<R> &x = a[c * kChunk + threadIdx.x];
+        x = mk<Amp<R>>(R(x.x * s), R(x.y * s));
+    }
+}
+
+}  // namespace
+
+void init(void *arena, bool c64, uint64_t total_amps, const uint64_t *d_aoff, int count, hipStream_t st)
+{
+    if (!c64) {
+        hipLaunchKernelGGL(k_zero<double>, grid_for(total_amps), dim3(kBlock), 0, st, static_cast<double2 *>(arena), total_amps);
+        hipLaunchKernelGGL(k_set_first<double>, dim3((count + 255) / 256), dim3(256), 0, st, static_cast<double2 *>(arena),
+                           d_aoff, count);
+    } else {
+        hipLaunchKernelGGL(k_zero<float>, grid_for(total_amps), dim3(kBlock), 0, st, static_cast<float2 *>(arena), total_amps);
+        hipLaunchKernelGGL(k_set_first<float>, dim3((count + 255) / 256), dim3(256), 0, st, static_cast<float2 *>(arena),
+                           d_aoff, count);
+    }
+}
+
+void norms(const void *arena, bool c64, uint64_t total_amps, const uint64_t *d_chunk_begin, int count, double *part,
+           double *out, hipStream_t st)
+{
+    const uint64_t nchunks = total_amps / kChunk;
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(nchunks, 8192)));
+    if (!c64) hipLaunchKernelGGL(k_chunk_norm<double>, g, dim3(kBlock), 0, st, static_cast<const double2 *>(arena), nchunks, part);
+    else hipLaunchKernelGGL(k_chunk_norm<float>, g, dim3(kBlock), 0, st, static_cast<const float2 *>(arena), nchunks, part);
+    hipLaunchKernelGGL(k_combine, dim3((count + 63) / 64), dim3(64), 0, st, part, 1, -1, d_chunk_begin, count, out);
+}
+
+void compare(const void *arena, bool c64, const void *ref_arena, uint64_t total_amps, const uint64_t *d_chunk_begin,
+             int count, double *part, double *out, hipStream_t st)
+{
+    const uint64_t nchunks = total_amps / kChunk;
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(nchunks, 8192)));
+    const double2 *ref = static_cast<const double2 *>(ref_arena);
+    if (!c64)
+        hipLaunchKernelGGL(k_chunk_compare<double>, g, dim3(kBlock), 0, st, static_cast<const double2 *>(arena), ref, nchunks,
+                           part);
+    else
+        hipLaunchKernelGGL(k_chunk_compare<float>, g, dim3(kBlock), 0, st, static_cast<const float2 *>(arena), ref, nchunks,
+                           part);
+    hipLaunchKernelGGL(k_combine, dim3((count + 63) / 64), dim3(64), 0, st, part, 5, 4, d_chunk_begin, count, out);
+}
+
+void scale(void *arena, bool c64, uint64_t total_amps, const uint64_t *d_chunk_begin, int count, const double *d_scale,
+           hipStream_t st)
+{
+    const uint64_t nchunks = total_amps / kChunk;
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(nchunks, 8192)));
+    if (!c64)
+        hipLaunchKernelGGL(k_scale_states<double>, g, dim3(kBlock), 0, st, static_cast<double2 *>(arena), nchunks,
+                           d_chunk_begin, count, d_scale);
+    else
+        hipLaunchKernelGGL(k_scale_states<float>, g, dim3(kBlock), 0, st, static_cast<float2 *>(arena), nchunks,
+                           d_chunk_begin, count, d_scale);
+}
+
+}  // namespace ev::batchk

@@ -9,10 +9,12 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "dm_ops.hpp"
 #include "engine.hpp"
 #include "fused.hpp"
 #include "planner.hpp"
+#include "server.hpp"
 
 struct ev_circuit {
     ev::Circuit c;
@@ -709,6 +711,119 @@ int ev_last_run_stats(ev_run_stats *out)
         out->pass_ms = s.pass_ms;
         out->pass_launches = s.pass_launches;
         out->plan_ms = s.plan_ms;
+    });
+}
+
+int ev_batch_run(const ev_circuit *const *circuits, size_t count, const ev_run_options *opt, ev_batch **out,
+                 ev_error *err)
+{
+    return guard(
+        [&] {
+            NONNULL(circuits);
+            NONNULL(out);
+            *out = nullptr;
+            ev_run_options o{};
+            if (opt) o = *opt;
+            (void)prec_of(o.precision);
+            std::vector<const ev::Circuit *> cs(count);
+            for (size_t i = 0; i < count; ++i) {
+                NONNULL(circuits[i]);
+                cs[i] = &circuits[i]->c;
+            }
+            *out = ev::batch_run(cs.data(), count, o);
+        },
+        err);
+}
+
+void ev_batch_destroy(ev_batch *b) { ev::batch_free(b); }
+size_t ev_batch_count(const ev_batch *b) { return b ? b->n.size() : 0; }
+uint64_t ev_batch_num_qubits(const ev_batch *b, size_t i) { return (b && i < b->n.size()) ? uint64_t(b->n[i]) : 0; }
+
+int ev_batch_read(const ev_batch *b, size_t i, uint64_t offset, ev_c128 *host, size_t len)
+{
+    return guard([&] {
+        NONNULL(b);
+        EV_REQUIRE(len == 0 || host != nullptr, EV_E_ARG, "host is null");
+        ev::batch_read(const_cast<ev_batch *>(b), i, offset, reinterpret_cast<ev::c128 *>(host), len);
+    });
+}
+
+int ev_batch_norms(const ev_batch *b, double *out)
+{
+    return guard([&] {
+        NONNULL(b);
+        NONNULL(out);
+        ev::batch_norms(const_cast<ev_batch *>(b), out);
+    });
+}
+
+int ev_batch_compare(const ev_batch *b, const ev_batch *ref, double *fidelity, double *max_abs)
+{
+    return guard([&] {
+        NONNULL(b);
+        NONNULL(ref);
+        NONNULL(fidelity);
+        NONNULL(max_abs);
+        ev::batch_compare(const_cast<ev_batch *>(b), const_cast<ev_batch *>(ref), fidelity, max_abs);
+    });
+}
+
+int ev_batch_get_stats(const ev_batch *b, ev_batch_stats *out)
+{
+    return guard([&] {
+        NONNULL(b);
+        NONNULL(out);
+        *out = b->stats;
+    });
+}
+
+int ev_server_create(const ev_server_config *cfg, ev_server **out)
+{
+    return guard([&] {
+        NONNULL(out);
+        ev_server_config c{};
+        if (cfg) c = *cfg;
+        (void)prec_of(c.precision);
+        *out = ev::server_create(c);
+    });
+}
+
+void ev_server_destroy(ev_server *s) { ev::server_destroy(s); }
+
+int ev_server_submit(ev_server *s, const ev_circuit *c, uint64_t shots, uint64_t seed, uint64_t *request_id)
+{
+    return guard([&] {
+        NONNULL(s);
+        NONNULL(c);
+        NONNULL(request_id);
+        *request_id = ev::server_submit(s, c->c, shots, seed);
+    });
+}
+
+int ev_server_poll(ev_server *s, uint64_t request_id, int *done)
+{
+    return guard([&] {
+        NONNULL(s);
+        NONNULL(done);
+        *done = ev::server_poll(s, request_id) ? 1 : 0;
+    });
+}
+
+int ev_server_wait(ev_server *s, uint64_t request_id, uint64_t *samples, ev_request_info *info)
+{
+    return guard([&] {
+        NONNULL(s);
+        const ev_request_info r = ev::server_wait(s, request_id, samples);
+        if (info) *info = r;
+    });
+}
+
+int ev_server_get_stats(ev_server *s, ev_server_stats *out)
+{
+    return guard([&] {
+        NONNULL(s);
+        NONNULL(out);
+        *out = ev::server_stats(s);
     });
 }
 

@@ -11,6 +11,8 @@
 // reference's per-product rounding to ~1e-15 relative per gate.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "device.hpp"
 #include "fused.hpp"
@@ -182,21 +184,21 @@ __device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfi
     }
 }
 
+// One tile of one pass.  Threads beyond the tile's 2^(t-4) (a batch launch
+// sizes its workgroups for the largest tile) skip the data movement but keep
+// every barrier.
 template <class R>
-__global__ __launch_bounds__(kMaxThreads) void k_fused_pass(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
-                                                    const Stage *__restrict__ stages, const DevOp *__restrict__ ops,
-                                                    const Amp<R> *__restrict__ coef)
+__device__ __forceinline__ void run_tile(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
+                                         const Stage *__restrict__ stages, const DevOp *__restrict__ ops,
+                                         const Amp<R> *__restrict__ coef, uint64_t T, Amp<R> *lds)
 {
     using V = Amp<R>;
-    extern __shared__ unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
-
     const int t = pass->t;
     const uint32_t tid = threadIdx.x;
     const uint32_t W = 1u << (t - kRegBits);
+    const bool active = tid < W;
 
-    // Tile base: deposit the workgroup id into the non-tile qubits.
-    const uint64_t T = blockIdx.x;
+    // Tile base: deposit the tile index into the non-tile qubits.
     uint64_t base = 0;
     for (int i = 0; i < pass->ncomp; ++i) base |= ((T >> i) & 1ull) << pass->comp[i];
 
@@ -204,43 +206,80 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_pass(Amp<R> *__restrict__
     uint64_t gthr = 0;
     for (int i = 0; i < t - kRegBits; ++i)
         if ((tid >> i) & 1u) gthr |= 1ull << pass->q[i];
+    if (active) {
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) {
-        uint64_t gj = 0;
+        for (int j = 0; j < kRegs; ++j) {
+            uint64_t gj = 0;
 #pragma unroll
-        for (int k = 0; k < kRegBits; ++k)
-            if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
-        lds[swz(uint32_t(j) * W + tid)] = psi[base | gthr | gj];
+            for (int k = 0; k < kRegBits; ++k)
+                if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
+            lds[swz(uint32_t(j) * W + tid)] = psi[base | gthr | gj];
+        }
     }
     __syncthreads();
 
     const int nst = pass->nstages;
     for (int s = 0; s < nst; ++s) {
         const Stage &st = stages[pass->stage_begin + s];
-        uint32_t lb = 0;
-        uint64_t gfix = base;
-        for (int i = 0; i < st.np; ++i)
-            if ((tid >> i) & 1u) {
-                lb |= 1u << st.p[i];
-                gfix |= 1ull << st.gpos[i];
-            }
-        const uint32_t sb = swz(lb);
-        Regs<V> v;
+        if (active) {
+            uint32_t lb = 0;
+            uint64_t gfix = base;
+            for (int i = 0; i < st.np; ++i)
+                if ((tid >> i) & 1u) {
+                    lb |= 1u << st.p[i];
+                    gfix |= 1ull << st.gpos[i];
+                }
+            const uint32_t sb = swz(lb);
+            Regs<V> v;
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) v[j] = lds[sb ^ st.roff[j]];
-        for (int o = st.op_begin; o < st.op_end; ++o) run_op<V>(ops[o], v, gfix, coef);
+            for (int j = 0; j < kRegs; ++j) v[j] = lds[sb ^ st.roff[j]];
+            for (int o = st.op_begin; o < st.op_end; ++o) run_op<V>(ops[o], v, gfix, coef);
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) lds[sb ^ st.roff[j]] = v[j];
+            for (int j = 0; j < kRegs; ++j) lds[sb ^ st.roff[j]] = v[j];
+        }
         __syncthreads();
     }
 
+    if (active) {
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) {
-        uint64_t gj = 0;
+        for (int j = 0; j < kRegs; ++j) {
+            uint64_t gj = 0;
 #pragma unroll
-        for (int k = 0; k < kRegBits; ++k)
-            if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
-        psi[base | gthr | gj] = lds[swz(uint32_t(j) * W + tid)];
+            for (int k = 0; k < kRegBits; ++k)
+                if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
+            psi[base | gthr | gj] = lds[swz(uint32_t(j) * W + tid)];
+        }
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kMaxThreads) void k_fused_pass(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
+                                                            const Stage *__restrict__ stages,
+                                                            const DevOp *__restrict__ ops,
+                                                            const Amp<R> *__restrict__ coef)
+{
+    extern __shared__ unsigned char lds_raw[];
+    run_tile<R>(psi, pass, stages, ops, coef, blockIdx.x, reinterpret_cast<Amp<R> *>(lds_raw));
+}
+
+// Persistent batch kernel: linear tile ids over every entry of the round
+// (entries sorted by tile_begin); workgroups stride over them.
+template <class R>
+__global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *__restrict__ entries, int nentries,
+                                                             uint64_t total_tiles)
+{
+    extern __shared__ unsigned char lds_raw[];
+    Amp<R> *lds = reinterpret_cast<Amp<R> *>(lds_raw);
+    for (uint64_t lin = blockIdx.x; lin < total_tiles; lin += gridDim.x) {
+        int lo = 0, hi = nentries - 1;  // last entry with tile_begin <= lin
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (entries[mid].tile_begin <= lin) lo = mid; else hi = mid - 1;
+        }
+        const BatchEntry &e = entries[lo];
+        run_tile<R>(static_cast<Amp<R> *>(e.psi), e.pass, e.stages, e.ops, static_cast<const Amp<R> *>(e.coef),
+                    lin - e.tile_begin, lds);
+        __syncthreads();  // the next tile's load overwrites this tile's LDS
     }
 }
 
@@ -261,6 +300,21 @@ void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stag
         hipLaunchKernelGGL(k_fused_pass<float>, grid, block, lds, st, static_cast<float2 *>(psi), pass, stages, ops,
                            static_cast<const float2 *>(coef));
     }
+}
+
+void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, unsigned grid,
+                  hipStream_t st)
+{
+    EV_REQUIRE(max_t >= kMinTile && max_t <= kMaxTile, EV_E_ARG, "batch tile width out of range");
+    if (nentries == 0 || total_tiles == 0) return;
+    const dim3 block(1u << (max_t - kRegBits));
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(grid, total_tiles)));
+    if (!c64)
+        hipLaunchKernelGGL(k_fused_batch<double>, g, block, (size_t(1) << max_t) * sizeof(double2), st, entries, nentries,
+                           total_tiles);
+    else
+        hipLaunchKernelGGL(k_fused_batch<float>, g, block, (size_t(1) << max_t) * sizeof(float2), st, entries, nentries,
+                           total_tiles);
 }
 
 }  // namespace ev::fused

@@ -13,4 +13,20 @@ namespace ev::fused {
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
                  const void *coef, hipStream_t st);
 
+// One entry of a batched round: the next pass of one state.  tile_begin is the
+// prefix sum of the tile counts of the entries before it.
+struct BatchEntry {
+    void *psi;
+    const Pass *pass;
+    const Stage *stages;
+    const DevOp *ops;
+    const void *coef;
+    uint64_t tile_begin;
+};
+
+// One persistent launch over every entry of a round: `grid` workgroups of
+// 2^(max_t-4) threads stride over the total_tiles linear tile ids.
+void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, unsigned grid,
+                  hipStream_t st);
+
 }  // namespace ev::fused

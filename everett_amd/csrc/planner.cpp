@@ -171,13 +171,6 @@ class Lowering {
     std::vector<bool> has_;
 };
 
-struct Plan {
-    std::vector<Pass> passes;
-    std::vector<Stage> stages;
-    std::vector<DevOp> devops;
-    std::vector<c128> coef;
-};
-
 int popc(uint64_t x) { return __builtin_popcountll(x); }
 
 int rank_in(uint64_t mask, int bit) { return popc(mask & ((uint64_t(1) << bit) - 1)); }
@@ -368,10 +361,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-struct FusedConfig {
-    int t, low;
-    float budget;
-};
+}  // namespace
 
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 {
@@ -385,6 +375,8 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
     f.budget = opt.max_gates_per_pass ? float(opt.max_gates_per_pass) : 20.0f;
     return f;
 }
+
+namespace {
 
 // Uploads a plan and launches its passes on s's stream.
 void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
@@ -445,7 +437,8 @@ void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st
     low.flush_all();
     if (low.ops.empty()) return;
     const double t0 = now_ms();
-    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, cfg.budget);
+    const float budget = (cfg.t >= s->n) ? 1e30f : cfg.budget;  // one tile holds the state: one pass
+    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget);
     st.plan_ms += now_ms() - t0;
     run_plan(s, P, st, prof);
     low.ops.clear();
@@ -493,11 +486,9 @@ void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, c
     flush_segment(s, low, cfg, st, prof);
 }
 
-PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)
+Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg)
 {
-    const int n = int(c.num_qubits());
     EV_REQUIRE(n >= fused::kMinTile, EV_E_ARG, "fused plans need at least 8 qubits");
-    const FusedConfig cfg = fused_config(n, opt.precision == EV_C64, opt);
     Lowering low(n);
     for (size_t i = 0; i < c.size(); ++i) {
         const Op &op = c.ops()[i];
@@ -505,11 +496,19 @@ PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)
         case OpKind::Apply1: low.gate1(int(op.q0), c.mat(op)); break;
         case OpKind::Apply2: low.gate2(int(op.q0), int(op.q1), c.mat(op)); break;
         case OpKind::Controlled: low.controlled(c.ctrls(op), int(op.nctrl), int(op.q0), c.mat(op)); break;
-        default: throw Error(EV_E_ARG, "debug_plan takes measurement-free circuits");
+        default: throw Error(EV_E_ARG, "fused plans take measurement-free circuits");
         }
     }
     low.flush_all();
-    const Plan P = plan(low.ops, low.pool, n, cfg.t, cfg.low, cfg.budget);
+    const float budget = (cfg.t >= n) ? 1e30f : cfg.budget;
+    return plan(low.ops, low.pool, n, cfg.t, cfg.low, budget);
+}
+
+PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)
+{
+    const int n = int(c.num_qubits());
+    const FusedConfig cfg = fused_config(n, opt.precision == EV_C64, opt);
+    const Plan P = plan_circuit(c, n, cfg);
     PlanBlob b;
     auto bytes = [](const void *p, size_t len) {
         const unsigned char *c = static_cast<const unsigned char *>(p);

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "fused.hpp"
 
 namespace ev {
 
@@ -13,6 +14,28 @@ namespace ev {
 // executor.
 void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const ev_run_options &opt,
                      RunStats &st, Profiler &prof);
+
+// Fused pass configuration: tile width t, low qubits always in the tile, and
+// the per-pass compute budget in units of one dense 2x2 over the tile.
+struct FusedConfig {
+    int t, low;
+    float budget;
+};
+FusedConfig fused_config(int n, bool c64, const ev_run_options &opt);
+
+// A planned gate segment: passes, their stages, the device ops and the c128
+// coefficient pool they index.
+struct Plan {
+    std::vector<fused::Pass> passes;
+    std::vector<fused::Stage> stages;
+    std::vector<fused::DevOp> devops;
+    std::vector<c128> coef;
+};
+
+// Plans every gate of a measurement-free circuit for an n-qubit state.  When
+// the tile covers the whole state (t == n) the budget is lifted: the circuit
+// is one pass.
+Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg);
 
 // Plans a measurement-free circuit without executing it (introspection).
 struct PlanBlob {

@@ -1,0 +1,425 @@
+// Continuous-batching circuit server (ev_server_* in include/everett.h).
+//
+// submit() plans the circuit on the caller's thread (fused passes, as for a
+// single run) and queues it.  The scheduler thread:
+//   1. admits queued requests in arrival order while their statevectors fit
+//      the HBM budget: allocates the state (from a size-keyed cache of device
+//      blocks), uploads the plan, resets the state to |0..0>;
+//   2. runs one step: ONE persistent launch over the next pass of every
+//      admitted request (the batch kernel), so a 28-qubit request and a
+//      handful of 12-qubit ones share each launch;
+//   3. samples requests whose last pass ran (f64 CDF, host-drawn uniforms
+//      from Rng::seed_from_u64(seed)), releases their memory and wakes waiters.
+// New requests join at the next step; nothing waits for the batch to drain.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+#include "fused_kernels.hpp"
+#include "kernels.hpp"
+#include "planner.hpp"
+
+namespace ev {
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Request {
+    uint64_t id = 0;
+    int n = 0;
+    uint64_t shots = 0, seed = 0, gates = 0;
+    Plan plan;
+    size_t state_bytes = 0;
+    void *psi = nullptr;
+    void *desc = nullptr;
+    size_t desc_bytes = 0;
+    size_t pass_off = 0, stage_off = 0, op_off = 0, coef_off = 0;
+    size_t next_pass = 0;
+    std::vector<uint64_t> samples;
+    ev_request_info info{};
+    bool done = false;
+};
+
+// Device blocks cached by exact size: serving reuses the same few state sizes.
+class BlockCache {
+  public:
+    void *get(size_t bytes)
+    {
+        auto it = free_.find(bytes);
+        if (it != free_.end()) {
+            void *p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {  // give the cached blocks back and retry once
+            (void)hipGetLastError();
+            release_all();
+            EV_HIP(hipMalloc(&p, bytes));
+        }
+        return p;
+    }
+    void put(void *p, size_t bytes)
+    {
+        if (p) free_.emplace(bytes, p);
+    }
+    void release_all()
+    {
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+    }
+    ~BlockCache() { release_all(); }
+
+  private:
+    std::multimap<size_t, void *> free_;
+};
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace ev
+
+struct ev_server {
+    ev_server_config cfg{};
+    bool c64 = false;
+    size_t amp_bytes = 16;
+    uint64_t budget = 0;
+    hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<ev::Request>> queue;
+    std::vector<std::shared_ptr<ev::Request>> active;  // scheduler thread only
+    std::map<uint64_t, std::shared_ptr<ev::Request>> reqs;
+    bool stop = false;
+    uint64_t next_id = 1;
+    ev_server_stats st{};
+    ev::BlockCache cache;                              // scheduler thread only
+    ev::fused::BatchEntry *h_ent = nullptr;            // pinned
+    void *d_ent = nullptr;
+    size_t ent_cap = 0;
+    std::thread sched;
+};
+
+namespace ev {
+namespace {
+
+void fail(ev_server *s, const std::shared_ptr<Request> &r, int code)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->info.status = code;
+    r->info.done_ms = ms_since(s->t0);
+    r->done = true;
+    ++s->st.failed;
+    s->cv_done.notify_all();
+}
+
+void release(ev_server *s, Request &r)
+{
+    s->cache.put(r.psi, r.state_bytes);
+    s->cache.put(r.desc, r.desc_bytes);
+    r.psi = r.desc = nullptr;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.bytes_in_use -= r.state_bytes;
+}
+
+// Uploads the plan and resets the state (scheduler thread).
+void admit(ev_server *s, Request &r)
+{
+    const Plan &P = r.plan;
+    size_t off = 0;
+    r.pass_off = off;
+    off += align256(P.passes.size() * sizeof(fused::Pass));
+    r.stage_off = off;
+    off += align256(P.stages.size() * sizeof(fused::Stage));
+    r.op_off = off;
+    off += align256(std::max<size_t>(1, P.devops.size()) * sizeof(fused::DevOp));
+    r.coef_off = off;
+    off += align256(std::max<size_t>(1, P.coef.size()) * s->amp_bytes);
+    r.desc_bytes = off;
+    std::vector<unsigned char> host(off, 0);
+    std::memcpy(host.data() + r.pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
+    std::memcpy(host.data() + r.stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
+    if (!P.devops.empty()) std::memcpy(host.data() + r.op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
+    if (!s->c64) {
+        std::memcpy(host.data() + r.coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
+    } else {
+        float *f = reinterpret_cast<float *>(host.data() + r.coef_off);
+        for (size_t k = 0; k < P.coef.size(); ++k) {
+            f[2 * k] = float(P.coef[k].re);
+            f[2 * k + 1] = float(P.coef[k].im);
+        }
+    }
+    r.psi = s->cache.get(r.state_bytes);
+    r.desc = s->cache.get(r.desc_bytes);
+    EV_HIP(hipMemcpyAsync(r.desc, host.data(), off, hipMemcpyHostToDevice, s->stream));
+    kern::init_basis(r.psi, s->c64 ? kern::C64 : kern::C128, r.n, 0, s->stream);
+    EV_HIP(hipStreamSynchronize(s->stream));  // `host` is released on return
+}
+
+void finish(ev_server *s, const std::shared_ptr<Request> &r)
+{
+    // Born-rule shots: uniforms from the request's own stream, f64 CDF on the device.
+    Rng rng = Rng::seed_from_u64(r->seed);
+    std::vector<double> u(r->shots);
+    for (auto &x : u) x = rng.next_f64();
+    const size_t nscr = kern::sample_scratch_doubles(r->n);
+    const size_t b_scr = align256(nscr * sizeof(double)), b_u = align256(std::max<uint64_t>(1, r->shots) * 8);
+    const size_t total = b_scr + 2 * b_u;
+    unsigned char *scr = static_cast<unsigned char *>(s->cache.get(total));
+    r->samples.assign(r->shots, 0);
+    hipError_t e = hipSuccess;
+    if (r->shots) e = hipMemcpyAsync(scr + b_scr, u.data(), r->shots * 8, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        kern::sample(r->psi, s->c64 ? kern::C64 : kern::C128, r->n, reinterpret_cast<double *>(scr + b_scr), r->shots,
+                     reinterpret_cast<double *>(scr), reinterpret_cast<uint64_t *>(scr + b_scr + b_u), s->stream);
+        if (r->shots)
+            e = hipMemcpyAsync(r->samples.data(), scr + b_scr + b_u, r->shots * 8, hipMemcpyDeviceToHost, s->stream);
+    }
+    const hipError_t e2 = hipStreamSynchronize(s->stream);
+    s->cache.put(scr, total);
+    release(s, *r);
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->info.status = (e == hipSuccess && e2 == hipSuccess) ? EV_OK : EV_E_HIP;
+    r->info.done_ms = ms_since(s->t0);
+    r->done = true;
+    if (r->info.status == EV_OK) ++s->st.completed;
+    else ++s->st.failed;
+    s->cv_done.notify_all();
+}
+
+void step(ev_server *s)
+{
+    const size_t na = s->active.size();
+    if (na > s->ent_cap) {
+        if (s->h_ent) EV_HIP(hipHostFree(s->h_ent));
+        if (s->d_ent) EV_HIP(hipFree(s->d_ent));
+        s->h_ent = nullptr;
+        s->d_ent = nullptr;
+        const size_t cap = std::max<size_t>(64, 2 * na);
+        EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ent), cap * sizeof(fused::BatchEntry), hipHostMallocDefault));
+        EV_HIP(hipMalloc(&s->d_ent, cap * sizeof(fused::BatchEntry)));
+        s->ent_cap = cap;
+    }
+    uint64_t tiles = 0;
+    int max_t = fused::kMinTile;
+    for (size_t i = 0; i < na; ++i) {
+        Request &r = *s->active[i];
+        unsigned char *d = static_cast<unsigned char *>(r.desc);
+        const fused::Pass &ps = r.plan.passes[r.next_pass];
+        fused::BatchEntry &e = s->h_ent[i];
+        e.psi = r.psi;
+        e.pass = reinterpret_cast<const fused::Pass *>(d + r.pass_off) + r.next_pass;
+        e.stages = reinterpret_cast<const fused::Stage *>(d + r.stage_off);
+        e.ops = reinterpret_cast<const fused::DevOp *>(d + r.op_off);
+        e.coef = d + r.coef_off;
+        e.tile_begin = tiles;
+        tiles += uint64_t(1) << (r.n - ps.t);
+        max_t = std::max(max_t, int(ps.t));
+    }
+    hipEvent_t a, b;
+    EV_HIP(hipEventCreate(&a));
+    EV_HIP(hipEventCreate(&b));
+    hipError_t e = hipMemcpyAsync(s->d_ent, s->h_ent, na * sizeof(fused::BatchEntry), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(a, s->stream);
+    if (e == hipSuccess) {
+        const unsigned per_cu = unsigned(std::max<size_t>(1, (160u << 10) / ((size_t(1) << max_t) * s->amp_bytes)));
+        fused::launch_batch(s->c64, static_cast<const fused::BatchEntry *>(s->d_ent), int(na), tiles, max_t,
+                            256u * std::min(per_cu, 8u), s->stream);
+        e = hipEventRecord(b, s->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    EV_HIP(e);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->st.busy_ms += ms;
+        ++s->st.steps;
+    }
+    for (auto &r : s->active) ++r->next_pass;
+}
+
+void scheduler(ev_server *s)
+{
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> admitted;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->active.empty(); });
+            if (s->stop && s->queue.empty() && s->active.empty()) return;
+            // Admission in arrival order while the states fit the budget.
+            while (!s->queue.empty()) {
+                auto &r = s->queue.front();
+                if (s->cfg.max_active && s->active.size() + admitted.size() >= s->cfg.max_active) break;
+                if (s->st.bytes_in_use + r->state_bytes > s->budget) break;
+                s->st.bytes_in_use += r->state_bytes;
+                s->st.bytes_peak = std::max(s->st.bytes_peak, s->st.bytes_in_use);
+                r->info.admit_ms = ms_since(s->t0);
+                admitted.push_back(r);
+                s->queue.pop_front();
+            }
+            s->st.queued = s->queue.size();
+        }
+        for (auto &r : admitted) {
+            try {
+                admit(s, *r);
+                s->active.push_back(r);
+            } catch (const Error &e) {
+                release(s, *r);
+                fail(s, r, e.code);
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->st.active = s->active.size();
+        }
+        if (s->active.empty()) continue;
+        try {
+            step(s);
+        } catch (const Error &e) {
+            for (auto &r : s->active) {
+                release(s, *r);
+                fail(s, r, e.code);
+            }
+            s->active.clear();
+            continue;
+        }
+        std::vector<std::shared_ptr<Request>> still;
+        for (auto &r : s->active) {
+            if (r->next_pass < r->plan.passes.size()) {
+                still.push_back(r);
+                continue;
+            }
+            try {
+                finish(s, r);
+            } catch (const Error &e) {
+                release(s, *r);
+                fail(s, r, e.code);
+            }
+        }
+        s->active.swap(still);
+    }
+}
+
+}  // namespace
+
+ev_server *server_create(const ev_server_config &cfg)
+{
+    auto *s = new ev_server;
+    s->cfg = cfg;
+    s->c64 = cfg.precision == EV_C64;
+    s->amp_bytes = s->c64 ? 8 : 16;
+    s->t0 = std::chrono::steady_clock::now();
+    try {
+        EV_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        size_t free_b = 0, total_b = 0;
+        EV_HIP(hipMemGetInfo(&free_b, &total_b));
+        s->budget = cfg.memory_bytes ? cfg.memory_bytes : uint64_t(0.8 * double(free_b));
+    } catch (...) {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        throw;
+    }
+    s->sched = std::thread(scheduler, s);
+    return s;
+}
+
+void server_destroy(ev_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    (void)hipStreamSynchronize(s->stream);
+    if (s->h_ent) (void)hipHostFree(s->h_ent);
+    if (s->d_ent) (void)hipFree(s->d_ent);
+    s->cache.release_all();
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+uint64_t server_submit(ev_server *s, const Circuit &c, uint64_t shots, uint64_t seed)
+{
+    c.validate();
+    const int n = int(c.num_qubits());
+    EV_REQUIRE(n >= fused::kMinTile && n <= 30, EV_E_LIMIT, "served circuits need 8..30 qubits");
+    for (const Op &op : c.ops())
+        EV_REQUIRE(op.kind == OpKind::Apply1 || op.kind == OpKind::Apply2 || op.kind == OpKind::Controlled, EV_E_ARG,
+                   "served circuits must be measurement-free (shots are drawn at the end)");
+    auto r = std::make_shared<Request>();
+    r->n = n;
+    r->shots = shots;
+    r->seed = seed;
+    r->gates = c.size();
+    r->state_bytes = (size_t(1) << n) * s->amp_bytes;
+    if (r->state_bytes > s->budget)
+        throw Error(EV_E_ADMISSION, "a " + std::to_string(n) + "-qubit state exceeds the server's memory budget");
+    ev_run_options o{};
+    o.precision = s->cfg.precision;
+    o.tile_qubits = s->cfg.tile_qubits;
+    o.max_gates_per_pass = s->cfg.max_gates_per_pass;
+    r->plan = plan_circuit(c, n, fused_config(n, s->c64, o));  // on the caller's thread
+    std::lock_guard<std::mutex> lk(s->mu);
+    EV_REQUIRE(!s->stop, EV_E_STATE, "server is shutting down");
+    r->id = s->next_id++;
+    r->info.id = r->id;
+    r->info.num_qubits = uint64_t(n);
+    r->info.gates = r->gates;
+    r->info.passes = r->plan.passes.size();
+    r->info.shots = shots;
+    r->info.submit_ms = ms_since(s->t0);
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    ++s->st.submitted;
+    s->st.queued = s->queue.size();
+    s->cv_work.notify_all();
+    return r->id;
+}
+
+bool server_poll(ev_server *s, uint64_t id)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    EV_REQUIRE(it != s->reqs.end(), EV_E_ARG, "unknown request id");
+    return it->second->done;
+}
+
+ev_request_info server_wait(ev_server *s, uint64_t id, uint64_t *samples)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    EV_REQUIRE(it != s->reqs.end(), EV_E_ARG, "unknown request id");
+    auto r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (samples && !r->samples.empty()) std::memcpy(samples, r->samples.data(), r->samples.size() * sizeof(uint64_t));
+    s->reqs.erase(id);
+    return r->info;
+}
+
+ev_server_stats server_stats(ev_server *s)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->st;
+}
+
+}  // namespace ev

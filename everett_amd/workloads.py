@@ -64,3 +64,42 @@ def config2(n: int = 13, depth: int = 30):
         c.measure(q, q)
     ch = Channel.Custom(composite_noise_kraus())
     return c, NoiseModel(ch, ch, 0.01)
+
+
+def config3_circuit(i: int, first_seed: int = 1000) -> Circuit:
+    """Batched workload circuit i: Rng::seed_from_u64(first_seed + i) draws
+    n = 10 + next_u64() % 13 and depth = 10 + next_u64() % 51, then the
+    brick-wall layers continue on the same stream."""
+    rng = Rng.seed_from_u64(first_seed + i)
+    n = 10 + rng.next_u64() % 13
+    depth = 10 + rng.next_u64() % 51
+    return brickwall(n, depth, rng)
+
+
+def config3(count: int = 4096, first_seed: int = 1000):
+    return [config3_circuit(i, first_seed) for i in range(count)]
+
+
+def config4_request(r: int, seed: int = 4000):
+    """Serving request r: Rng::seed_from_u64(seed + r) draws
+    n = round(exp(ln 12 + (ln 28 - ln 12) * next_f64())) and
+    depth = 20 + next_u64() % 181, then brick-wall layers on the same stream.
+    Returns (circuit, shots seed)."""
+    import math
+
+    rng = Rng.seed_from_u64(seed + r)
+    n = int(round(math.exp(math.log(12) + (math.log(28) - math.log(12)) * rng.next_f64())))
+    depth = 20 + rng.next_u64() % 181
+    return brickwall(n, depth, rng), seed + r
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = 4000):
+    """Arrival times (s) with exponential gaps -ln(1-u)/rate from Rng::seed_from_u64(seed ^ 0xA5A5)."""
+    import math
+
+    rng = Rng.seed_from_u64(seed ^ 0xA5A5)
+    t, out = 0.0, []
+    for _ in range(count):
+        t += -math.log(1.0 - rng.next_f64()) / rate
+        out.append(t)
+    return out

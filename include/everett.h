@@ -158,7 +158,9 @@ typedef struct {
     uint32_t tile_qubits;   /* fused pass tile width; 0 = default */
     uint32_t max_gates_per_pass; /* 0 = default */
     uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
-    uint32_t reserved[3];
+    uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once */
+    uint32_t renormalize;   /* batch: rescale every state to unit f64 norm at the end (c64 drift) */
+    uint32_t reserved[1];
 } ev_run_options;
 
 /* src/backend/statevector.rs:68-88: validate, |0..0>, Rng::seed_from_u64(seed),
@@ -229,6 +231,75 @@ int ev_dm_partial_trace_1q(const ev_dm *d, uint64_t qubit, ev_c128 out[4]);     
  * NULL (NoiseModel::ideal, density_matrix.rs:76-78). */
 int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, const ev_run_options *opt,
               ev_dm **out, uint8_t *classical, size_t num_classical, ev_error *err);
+
+/* ------------------------------------------------------------------ batch */
+
+/* Many independent measurement-free circuits (8..30 qubits each), each from
+ * |0..0>, with all statevectors resident in one HBM arena.  Every circuit is
+ * planned into fused passes; round r is one persistent launch over the r-th
+ * pass of every circuit that has one (no reference counterpart: the
+ * reference runs one circuit per call, src/backend/statevector.rs:77-88). */
+typedef struct ev_batch ev_batch;
+typedef struct {
+    uint64_t circuits, logical_gates, rounds, passes, tiles;
+    double plan_ms, upload_ms, device_ms, renorm_ms;
+    uint32_t graph; /* 1 if the rounds ran as one captured HIP graph */
+    uint32_t pad;
+} ev_batch_stats;
+int ev_batch_run(const ev_circuit *const *circuits, size_t count, const ev_run_options *opt, ev_batch **out,
+                 ev_error *err);
+void ev_batch_destroy(ev_batch *b);
+size_t ev_batch_count(const ev_batch *b);
+uint64_t ev_batch_num_qubits(const ev_batch *b, size_t i);
+int ev_batch_read(const ev_batch *b, size_t i, uint64_t offset, ev_c128 *host, size_t len);
+/* f64 <psi_i|psi_i> for every state. */
+int ev_batch_norms(const ev_batch *b, double *out);
+/* Per state i: fidelity |<ref_i|b_i>|^2 / (<b_i|b_i><ref_i|ref_i>) and max-abs
+ * |b_i - ref_i|, against a c128 batch of the same circuits. */
+int ev_batch_compare(const ev_batch *b, const ev_batch *ref, double *fidelity, double *max_abs);
+int ev_batch_get_stats(const ev_batch *b, ev_batch_stats *out);
+
+/* ---------------------------------------------------------------- serving */
+
+/* A continuous-batching server (no reference counterpart).  Requests are
+ * measurement-free circuits ending in `shots` Born-rule samples.  A
+ * scheduler thread admits queued requests in arrival order while their
+ * statevectors fit the HBM budget (2^n * 8 B for c64, 2^n * 16 B for c128),
+ * then advances every admitted request by one fused pass per persistent
+ * launch, so requests of different sizes overlap on the device.  A request
+ * that completes is sampled (f64 CDF, one Rng::next_f64 per shot from
+ * Rng::seed_from_u64(seed)) and its memory is released.  Submit, poll and
+ * wait are thread-safe. */
+typedef struct ev_server ev_server;
+typedef struct {
+    ev_precision precision;
+    uint32_t max_active;        /* admitted requests at most (0 = no limit) */
+    uint64_t memory_bytes;      /* statevector budget (0 = 80% of free device memory) */
+    uint32_t tile_qubits;       /* 0 = default */
+    uint32_t max_gates_per_pass;/* 0 = default */
+    uint32_t reserved[4];
+} ev_server_config;
+typedef struct {
+    uint64_t id, num_qubits, gates, passes, shots;
+    double submit_ms, admit_ms, done_ms; /* server clock (ms since creation) */
+    int32_t status;                      /* EV_OK, or the error that ended the request */
+    int32_t pad;
+} ev_request_info;
+typedef struct {
+    uint64_t submitted, completed, failed, steps, active, queued;
+    uint64_t bytes_in_use, bytes_peak;
+    double busy_ms;   /* device time of the step launches */
+} ev_server_stats;
+int ev_server_create(const ev_server_config *cfg, ev_server **out);
+void ev_server_destroy(ev_server *s);   /* drains in-flight work, then frees */
+/* Copies the circuit; returns EV_E_ADMISSION if it can never fit the budget. */
+int ev_server_submit(ev_server *s, const ev_circuit *c, uint64_t shots, uint64_t seed, uint64_t *request_id);
+/* *done = 1 once the request has completed (or failed). */
+int ev_server_poll(ev_server *s, uint64_t request_id, int *done);
+/* Blocks until the request completes; copies its samples (shots entries) and
+ * info, then forgets the request. */
+int ev_server_wait(ev_server *s, uint64_t request_id, uint64_t *samples, ev_request_info *info);
+int ev_server_get_stats(ev_server *s, ev_server_stats *out);
 
 /* ---------------------------------------------------------- introspection */
 

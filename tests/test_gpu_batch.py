@@ -1,0 +1,74 @@
+"""Batched execution (many uneven statevectors, one launch per round) against
+the oracle and against single-circuit runs."""
+import numpy as np
+import pytest
+
+import everett_amd as ev
+import oracle
+from tests.test_planner import mixed_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch():
+    cs = []
+    for i in range(24):
+        rng = ev.Rng.seed_from_u64(500 + i)
+        n = 8 + rng.next_u64() % 7
+        cs.append(ev.workloads.brickwall(n, 3 + rng.next_u64() % 8, rng) if i % 3 else mixed_circuit(n, 120, i))
+    return cs
+
+
+def test_batch_c128_matches_oracle():
+    cs = small_batch()
+    b = ev.Batch.run(cs, precision="c128", use_graph=False)
+    assert len(b) == len(cs)
+    for i, c in enumerate(cs):
+        n = c.num_qubits()
+        want, _ = oracle.sv_run(c.ops(), n)
+        assert b.num_qubits(i) == n
+        assert np.max(np.abs(b.amplitudes(i) - want)) < 1e-10, i
+    st = b.stats()
+    assert st["circuits"] == len(cs) and st["logical_gates"] == sum(len(c) for c in cs)
+
+
+def test_batch_graph_equals_stream_launches():
+    cs = small_batch()
+    a = ev.Batch.run(cs, precision="c128", use_graph=False)
+    g = ev.Batch.run(cs, precision="c128", use_graph=True)
+    assert g.stats()["graph"] == 1
+    for i in range(len(cs)):
+        assert np.array_equal(a.amplitudes(i), g.amplitudes(i))
+
+
+def test_batch_c64_fidelity_and_renormalisation():
+    cs = small_batch()
+    ref = ev.Batch.run(cs, precision="c128")
+    x = ev.Batch.run(cs, precision="c64", renormalize=True)
+    f, m = x.compare(ref)
+    assert np.all(1 - f < 1e-6) and np.all(m < 1e-5)
+    assert np.max(np.abs(x.norms() - 1.0)) < 1e-6
+    assert np.max(np.abs(ref.norms() - 1.0)) < 1e-12
+
+
+def test_batch_uneven_sizes_like_config3():
+    cs = ev.workloads.config3(48)
+    ref = ev.Batch.run(cs, precision="c128")
+    x = ev.Batch.run(cs, precision="c64")
+    f, m = x.compare(ref)
+    assert np.all(1 - f < 1e-6) and np.all(m < 1e-5)
+    # the c128 batch equals single-circuit fused runs
+    for i in (0, 7, 31):
+        single = ev.StateVectorBackend.run(cs[i]).state().amplitudes()
+        assert np.max(np.abs(ref.amplitudes(i) - single)) < 1e-12
+    # and a small one against the oracle
+    i = int(np.argmin([c.num_qubits() for c in cs]))
+    want, _ = oracle.sv_run(cs[i].ops(), cs[i].num_qubits())
+    assert np.max(np.abs(ref.amplitudes(i) - want)) < 1e-10
+
+
+def test_batch_rejects_measurement():
+    c = ev.Circuit.with_classical(8, 1)
+    c.h(0).measure(0, 0)
+    with pytest.raises(ev.InvalidArgument):
+        ev.Batch.run([c])
