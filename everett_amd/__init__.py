@@ -16,9 +16,12 @@ from .backend import (
     Execution,
     RunStats,
     StateVectorBackend,
+    autotune,
     last_kernel_stats,
     last_run_stats,
     run_into,
+    tune_get,
+    tune_set,
 )
 from .circuit import Circuit
 from .errors import (
@@ -42,7 +45,7 @@ from .state import State, measure_qubit
 __all__ = [
     "algorithms", "kernel", "workloads", "Batch", "Server", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
     "StateVectorBackend", "Execution", "DensityMatrixBackend", "DensityMatrix", "DensityMatrixExecution",
-    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "EverettError", "QubitOutOfRange",
+    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "EverettError", "QubitOutOfRange",
     "ClassicalBitOutOfRange", "DuplicateQubit", "DimensionMismatch", "InvalidArgument", "DeviceError",
     "OutOfMemory", "AdmissionError", "LimitError", "StateError",
 ]

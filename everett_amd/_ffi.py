@@ -64,7 +64,7 @@ class EvRunOptions(C.Structure):
         ("profile", C.c_uint32),
         ("use_graph", C.c_uint32),
         ("renormalize", C.c_uint32),
-        ("reserved", C.c_uint32 * 1),
+        ("low_qubits", C.c_uint32),
     ]
 
 
@@ -115,6 +115,11 @@ class EvServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
                 ("active", C.c_uint64), ("queued", C.c_uint64), ("bytes_in_use", C.c_uint64),
                 ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double)]
+
+
+class EvTuneEntry(C.Structure):
+    _fields_ = [("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32), ("max_gates_per_pass", C.c_uint32),
+                ("pad", C.c_uint32), ("ms_per_gate", C.c_double)]
 
 
 class EvPlanInfo(C.Structure):
@@ -199,6 +204,9 @@ PROTOTYPES = {
     "ev_batch_norms": (I, [P, P]),
     "ev_batch_compare": (I, [P, P, P, P]),
     "ev_batch_get_stats": (I, [P, C.POINTER(EvBatchStats)]),
+    "ev_autotune": (I, [U64, I, C.c_uint32, C.POINTER(EvTuneEntry)]),
+    "ev_tune_get": (I, [U64, I, C.POINTER(EvTuneEntry), C.POINTER(I)]),
+    "ev_tune_set": (I, [U64, I, C.POINTER(EvTuneEntry)]),
     "ev_server_create": (I, [C.POINTER(EvServerConfig), PP]),
     "ev_server_destroy": (None, [P]),
     "ev_server_submit": (I, [P, P, U64, U64, C.POINTER(U64)]),

@@ -14,8 +14,9 @@ from .state import State, precision_code
 
 
 def run_options(precision="c128", plain: bool = False, tile_qubits: int = 0, max_gates_per_pass: int = 0,
-                profile: bool = False):
+                profile: bool = False, low_qubits: int = 0):
     o = _ffi.EvRunOptions()
+    o.low_qubits = low_qubits
     o.precision = precision_code(precision)
     o.plain = 1 if plain else 0
     o.tile_qubits = tile_qubits
@@ -93,8 +94,8 @@ class StateVectorBackend:
 
     @staticmethod
     def run_seeded(circuit: Circuit, seed: int, precision="c128", plain: bool = False, tile_qubits: int = 0,
-                   max_gates_per_pass: int = 0) -> Execution:
-        opt = run_options(precision, plain, tile_qubits, max_gates_per_pass)
+                   max_gates_per_pass: int = 0, low_qubits: int = 0) -> Execution:
+        opt = run_options(precision, plain, tile_qubits, max_gates_per_pass, low_qubits=low_qubits)
         ncl = circuit.num_classical()
         bits = np.zeros(max(ncl, 1), dtype=np.uint8)
         h = C.c_void_p()
@@ -217,3 +218,30 @@ class DensityMatrixBackend:
                                   _ffi.ptr(bits), ncl, C.byref(err))
         raise_for(rc, err)
         return DensityMatrixExecution(DensityMatrix(h), [bool(b) for b in bits[:ncl]])
+
+
+def autotune(n: int, precision="c128", depth: int = 0) -> dict:
+    """Times fused-pass parameters for n qubits and records the fastest (see ev_autotune)."""
+    e = _ffi.EvTuneEntry()
+    check(_ffi.lib().ev_autotune(n, precision_code(precision), depth, C.byref(e)))
+    return {"tile_qubits": e.tile_qubits, "low_qubits": e.low_qubits, "max_gates_per_pass": e.max_gates_per_pass,
+            "ms_per_gate": e.ms_per_gate}
+
+
+def tune_get(n: int, precision="c128"):
+    e = _ffi.EvTuneEntry()
+    found = C.c_int()
+    check(_ffi.lib().ev_tune_get(n, precision_code(precision), C.byref(e), C.byref(found)))
+    if not found.value:
+        return None
+    return {"tile_qubits": e.tile_qubits, "low_qubits": e.low_qubits, "max_gates_per_pass": e.max_gates_per_pass,
+            "ms_per_gate": e.ms_per_gate}
+
+
+def tune_set(n: int, precision, entry: dict) -> None:
+    e = _ffi.EvTuneEntry()
+    e.tile_qubits = entry["tile_qubits"]
+    e.low_qubits = entry["low_qubits"]
+    e.max_gates_per_pass = entry["max_gates_per_pass"]
+    e.ms_per_gate = entry.get("ms_per_gate", 0.0)
+    check(_ffi.lib().ev_tune_set(n, precision_code(precision), C.byref(e)))

@@ -15,6 +15,7 @@
 #include "fused.hpp"
 #include "planner.hpp"
 #include "server.hpp"
+#include "tune.hpp"
 
 struct ev_circuit {
     ev::Circuit c;
@@ -824,6 +825,34 @@ int ev_server_get_stats(ev_server *s, ev_server_stats *out)
         NONNULL(s);
         NONNULL(out);
         *out = ev::server_stats(s);
+    });
+}
+
+int ev_autotune(uint64_t n, ev_precision p, uint32_t depth, ev_tune_entry *out)
+{
+    return guard([&] {
+        (void)prec_of(p);
+        const ev_tune_entry e = ev::autotune(int(n), p == EV_C64, int(depth));
+        if (out) *out = e;
+    });
+}
+
+int ev_tune_get(uint64_t n, ev_precision p, ev_tune_entry *out, int *found)
+{
+    return guard([&] {
+        NONNULL(out);
+        NONNULL(found);
+        (void)prec_of(p);
+        *found = ev::tune_lookup(int(n), p == EV_C64, out) ? 1 : 0;
+    });
+}
+
+int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e)
+{
+    return guard([&] {
+        NONNULL(e);
+        (void)prec_of(p);
+        ev::tune_store(int(n), p == EV_C64, *e);
     });
 }
 

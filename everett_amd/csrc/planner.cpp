@@ -26,6 +26,7 @@
 #include "dm_ops.hpp"
 #include "fused.hpp"
 #include "fused_kernels.hpp"
+#include "tune.hpp"
 
 namespace ev {
 
@@ -367,12 +368,23 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 {
     FusedConfig f;
     const int tmax = fused::kMaxTile;  // 256 threads x 16 amplitudes
-    f.t = opt.tile_qubits ? int(opt.tile_qubits) : tmax;
-    EV_REQUIRE(f.t >= fused::kMinTile && f.t <= tmax, EV_E_ARG,
-               "tile_qubits must be in [8, " + std::to_string(tmax) + "] for this precision");
-    f.t = std::min(f.t, n);
+    f.t = tmax;
     f.low = c64 ? 5 : 4;  // 256-byte contiguous rows
-    f.budget = opt.max_gates_per_pass ? float(opt.max_gates_per_pass) : 20.0f;
+    f.budget = 20.0f;
+    ev_tune_entry te;
+    if (!opt.tile_qubits && !opt.low_qubits && !opt.max_gates_per_pass && tune_lookup(n, c64, &te)) {
+        f.t = int(te.tile_qubits);
+        f.low = int(te.low_qubits);
+        f.budget = float(te.max_gates_per_pass);
+    }
+    if (opt.tile_qubits) f.t = int(opt.tile_qubits);
+    if (opt.low_qubits) f.low = int(opt.low_qubits);
+    if (opt.max_gates_per_pass) f.budget = float(opt.max_gates_per_pass);
+    EV_REQUIRE(f.t >= fused::kMinTile && f.t <= tmax, EV_E_ARG,
+               "tile_qubits must be in [8, " + std::to_string(tmax) + "]");
+    EV_REQUIRE(f.low >= 1 && f.low <= 6, EV_E_ARG, "low_qubits must be in [1, 6]");
+    f.t = std::min(f.t, n);
+    f.low = std::min(f.low, f.t - 1);
     return f;
 }
 

@@ -160,7 +160,7 @@ typedef struct {
     uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
     uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once */
     uint32_t renormalize;   /* batch: rescale every state to unit f64 norm at the end (c64 drift) */
-    uint32_t reserved[1];
+    uint32_t low_qubits;    /* lowest qubits always in a tile (contiguous rows); 0 = tuned or default */
 } ev_run_options;
 
 /* src/backend/statevector.rs:68-88: validate, |0..0>, Rng::seed_from_u64(seed),
@@ -300,6 +300,23 @@ int ev_server_poll(ev_server *s, uint64_t request_id, int *done);
  * info, then forgets the request. */
 int ev_server_wait(ev_server *s, uint64_t request_id, uint64_t *samples, ev_request_info *info);
 int ev_server_get_stats(ev_server *s, ev_server_stats *out);
+
+/* ------------------------------------------------------------- autotuner */
+
+/* Fused-pass parameters for one state size: tile width, contiguous low
+ * qubits per tile and gate budget per pass, with the measured cost. */
+typedef struct {
+    uint32_t tile_qubits, low_qubits, max_gates_per_pass, pad;
+    double ms_per_gate;
+} ev_tune_entry;
+/* Times the fused path on a synthetic brick-wall circuit of n qubits (depth
+ * `depth`, 0 = 8) over candidate budgets, tile widths and low-qubit counts
+ * (coordinate descent) and records the fastest for (n, precision): runs that
+ * leave tile_qubits, low_qubits and max_gates_per_pass at 0 use it. */
+int ev_autotune(uint64_t n, ev_precision p, uint32_t depth, ev_tune_entry *out);
+/* Reads or sets the recorded entry (e.g. restored from a file). */
+int ev_tune_get(uint64_t n, ev_precision p, ev_tune_entry *out, int *found);
+int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e);
 
 /* ---------------------------------------------------------- introspection */
 

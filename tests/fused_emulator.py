@@ -26,10 +26,10 @@ OP_DTYPE = np.dtype([("kind", "u1"), ("b0", "u1"), ("b1", "u1"), ("flags", "u1")
 U1, CU1, CX, DIAGR, DIAGT, U2, SWAP = range(7)
 
 
-def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0):
+def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low_qubits=0):
     from everett_amd.backend import run_options
 
-    opt = run_options(precision, False, tile_qubits, max_gates_per_pass)
+    opt = run_options(precision, False, tile_qubits, max_gates_per_pass, low_qubits=low_qubits)
     info = _ffi.EvPlanInfo()
     check(_ffi.lib().ev_debug_plan(circuit.handle, C.byref(opt), None, None, None, None, C.byref(info)))
     assert info.pass_bytes == PASS_DTYPE.itemsize

@@ -195,3 +195,35 @@ def test_workload_sizes():
     assert len(ev.workloads.config1()) == 4660
     c2, noise = ev.workloads.config2()
     assert c2.num_qubits() == 13 and c2.num_classical() == 13 and noise.readout_error == 0.01
+
+
+def test_tune_table_roundtrip_and_plan_uses_it():
+    from tests.fused_emulator import get_plan
+
+    assert ev.tune_get(17, "c64") is None
+    ev.tune_set(17, "c64", {"tile_qubits": 9, "low_qubits": 3, "max_gates_per_pass": 7})
+    e = ev.tune_get(17, "c64")
+    assert (e["tile_qubits"], e["low_qubits"], e["max_gates_per_pass"]) == (9, 3, 7)
+    c = ev.workloads.config0(17, 3)
+    _, _, _, _, t = get_plan(c, precision="c64")
+    assert t == 9
+    _, _, _, _, t = get_plan(c, precision="c64", tile_qubits=11)  # explicit options win
+    assert t == 11
+    with pytest.raises(ev.InvalidArgument):
+        ev.tune_set(17, "c64", {"tile_qubits": 13, "low_qubits": 3, "max_gates_per_pass": 7})
+
+
+@pytest.mark.parametrize("low", [1, 2, 3, 5, 6])
+def test_low_qubit_option_in_plans(low):
+    from tests.fused_emulator import check_plan, emulate, get_plan
+
+    c = ev.workloads.config1(10, 5)
+    p, s, o, cf, t = get_plan(c, tile_qubits=8, low_qubits=low)
+    check_plan(10, p, s, o, cf)
+    for P in p:
+        assert list(P["q"][:low]) == list(range(low))
+    rng = np.random.default_rng(low)
+    psi = rng.standard_normal(1 << 10) + 1j * rng.standard_normal(1 << 10)
+    want = psi.copy()
+    oracle.sv_apply_ops(c.ops(), want)
+    assert np.max(np.abs(emulate(psi, 10, p, s, o, cf) - want)) < 1e-12
