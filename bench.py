@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark: gates/s and HBM GB/s of a 30-qubit complex128 random circuit on one MI355X.
+"""Benchmark: gates/s and HBM GB/s of a 30-qubit complex128 circuit on one MI355X,
+plus the batched, served, density-matrix and 20-qubit workloads.
 
-Workload (default): a 30-qubit brick-wall circuit of depth 40 (Rx, Ry, Rz on
-every qubit per layer, angles TAU * Rng::next_f64() from seed 30, then CNOTs
-on alternating pairs) followed by QFT(30): 4,660 logical gates.  One step =
-reset the HBM-resident state to |0..0> and run the whole circuit.  The state
-is allocated once, outside the timed region.
+Headline (c1): a 30-qubit brick-wall circuit of depth 40 (Rx, Ry, Rz on every
+qubit per layer, angles TAU * Rng::next_f64() from seed 30, then CNOTs on
+alternating pairs) followed by QFT(30): 4,660 logical gates.  One step =
+reset the HBM-resident state to |0..0> and run the whole circuit; the state is
+allocated once, outside the timed region.  `value` is gates/s over the timed
+steps; `roofline` is the dominant kernel's algorithmic bytes per launch over
+its average launch time (HIP events on the library's stream around every
+launch in the timed steps); `cpu_baseline` is the oracle (a single-threaded C
+restatement of the reference's loops) on a bounded prefix of the same circuit.
 
-Prints one JSON line with the rate, the roofline of the dominant kernel
-(algorithmic bytes per launch / average launch time, HIP events on the
-library's stream over the timed steps) and a CPU baseline: the oracle (a
-single-threaded C restatement of the reference's loops) timed on this host
-over a bounded prefix of the same workload.
+Extra workloads in the same JSON line (see docs/configs.md):
+  c0  20-qubit brick-wall depth 20, complex128
+  c2  13-qubit noisy density matrix, depth 30, readout error 1%
+  c3  4096 uneven statevectors (10..22 qubits), complex64, one launch per round
+  c4  Poisson-arrival serving, 12..28 qubits, 1024 shots each, p50/p99 latency
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -29,21 +36,11 @@ METRIC = "gates/sec + HBM GB/s at 30q f64; served circuits/sec with p50/p99 late
 HBM_PEAK_GBPS = 8000.0
 
 
-def workload(name: str, n: int | None):
-    import everett_amd as ev
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    if name == "c1":
-        n = n or 30
-        c = ev.workloads.config1(n, 40)
-        desc = f"{n}-qubit brick-wall depth 40 (seed {n}) + QFT({n})"
-    elif name == "c0":
-        n = n or 20
-        c = ev.workloads.config0(n, 20)
-        desc = f"{n}-qubit brick-wall depth 20 (seed {n})"
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return c, n, desc
 
+# ------------------------------------------------------------------ c1 / c0
 
 def cpu_baseline(circuit, n: int, seconds: float) -> dict:
     """Times the oracle (1 thread) on a prefix of the same circuit at the same n."""
@@ -74,17 +71,257 @@ def cpu_baseline(circuit, n: int, seconds: float) -> dict:
     }
 
 
+def run_sv_workload(circuit, n, precision, steps, warmup, plain):
+    import everett_amd as ev
+
+    t = time.perf_counter()
+    state = ev.State.zero(n, precision)
+    alloc_ms = (time.perf_counter() - t) * 1e3
+    for _ in range(warmup):
+        ev.run_into(state, circuit, 0, plain=plain)
+    state.norm_sqr()  # synchronises the handle's stream
+    agg: dict = {}
+    dev_ms = 0.0
+    passes = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.run_into(state, circuit, 0, plain=plain, profile=True)
+        rs = ev.last_run_stats()
+        dev_ms += rs.device_ms
+        passes += rs.passes
+        for ks in ev.last_kernel_stats():
+            a = agg.setdefault(ks["name"], {"launches": 0, "total_ms": 0.0, "bytes": 0.0})
+            a["launches"] += ks["launches"]
+            a["total_ms"] += ks["total_ms"]
+            a["bytes"] += ks["bytes"]
+    norm = state.norm_sqr()
+    wall = time.perf_counter() - t0
+    del state
+    if abs(norm - 1.0) > 1e-6:
+        raise SystemExit(f"norm drifted to {norm}")
+    return {"wall": wall, "agg": agg, "dev_ms": dev_ms, "alloc_ms": alloc_ms, "passes": passes / max(steps, 1)}
+
+
+def roofline(agg, steps, workload_key, n, precision):
+    dom_name, dom = max(agg.items(), key=lambda kv: kv[1]["total_ms"])
+    per_launch = dom["bytes"] / dom["launches"]
+    avg_ms = dom["total_ms"] / dom["launches"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        entry = json.loads(tf.read_text()).get(f"{dom_name}:{workload_key}:{n}:{precision}")
+        if entry:
+            traffic = entry.get("hbm_bytes_per_launch")
+    return {
+        "bound": "hbm",
+        "kernel": dom_name,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBPS,
+        "traffic": traffic,
+        "bytes_per_launch": per_launch,
+        "avg_launch_ms": avg_ms,
+        "launches_per_step": dom["launches"] / steps,
+    }
+
+
+def bench_sv(name, circuit, n, desc, args, steps, warmup):
+    r = run_sv_workload(circuit, n, args.precision, steps, warmup, args.plain)
+    gates = len(circuit)
+    step_bytes = sum(a["bytes"] for a in r["agg"].values()) / steps
+    return {
+        "gates": gates,
+        "gates_per_s": gates * steps / r["wall"],
+        "ms_per_step": r["wall"] / steps * 1e3,
+        "device_ms_per_step": r["dev_ms"] / steps,
+        "passes_per_step": r["passes"],
+        "hbm_gbps": step_bytes / (r["wall"] / steps) / 1e9,
+        "state_alloc_ms": r["alloc_ms"],
+        "roofline": roofline(r["agg"], steps, name, n, args.precision),
+        "kernels": {k: {"launches_per_step": v["launches"] / steps, "ms_per_step": v["total_ms"] / steps,
+                        "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["total_ms"] else None}
+                    for k, v in sorted(r["agg"].items(), key=lambda kv: -kv[1]["total_ms"])},
+        "workload": desc,
+    }
+
+
+# ----------------------------------------------------------------------- c2
+
+def bench_c2(args) -> dict:
+    import everett_amd as ev
+
+    c, noise = ev.workloads.config2(13, 30)
+    ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 13)  # warmup
+    t0 = time.perf_counter()
+    ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 13)
+    rho = ex.density_matrix()
+    tr = rho.trace()
+    wall = time.perf_counter() - t0
+    rs = ev.last_run_stats()
+    gates = rs.logical_gates
+    pass_bytes = 2 * (4 ** 13) * 16
+    return {
+        "workload": "13-qubit noisy density matrix: dm_random depth 30 (seed 13), depolarizing 1e-3 then amplitude "
+                    "damping 5e-4 after every gate, readout error 1%, all qubits measured",
+        "gates": int(gates),
+        "gates_per_s": gates / wall,
+        "ms": wall * 1e3,
+        "device_ms": rs.device_ms,
+        "superop_passes": int(rs.passes),
+        "gbps_per_pass": (pass_bytes * rs.passes) / (rs.device_ms * 1e-3) / 1e9 if rs.device_ms else None,
+        "trace": tr,
+        "classical": "".join("1" if b else "0" for b in ex.classical()),
+    }
+
+
+# ----------------------------------------------------------------------- c3
+
+def bench_c3(args) -> dict:
+    import numpy as np
+
+    import everett_amd as ev
+
+    count = args.batch_count
+    cs = ev.workloads.config3(count)
+    gates = sum(len(c) for c in cs)
+    sizes = sorted({c.num_qubits() for c in cs})
+    t = time.perf_counter()
+    for n in sizes:
+        if ev.tune_get(n, "c64") is None:
+            ev.autotune(n, "c64", 8)
+    tune_s = time.perf_counter() - t
+    ev.Batch.run(cs[:64], precision="c64")  # warmup
+    t0 = time.perf_counter()
+    x = ev.Batch.run(cs, precision="c64", use_graph=True, renormalize=True)
+    wall = time.perf_counter() - t0
+    st = x.stats()
+    ref = ev.Batch.run(cs, precision="c128", use_graph=True)
+    f, m = x.compare(ref)
+    infid = 1.0 - f
+    del ref, x
+    return {
+        "workload": f"{count} uneven brick-wall circuits (n 10..22, depth 10..60, seeds 1000+i), complex64 bulk, "
+                    "f64 norms and renormalisation, one persistent launch per round, HIP graph",
+        "circuits": count,
+        "gates": gates,
+        "circuits_per_s": count / wall,
+        "gates_per_s": gates / wall,
+        "device_circuits_per_s": count / (st["device_ms"] * 1e-3),
+        "wall_ms": wall * 1e3,
+        "plan_ms": st["plan_ms"],
+        "upload_ms": st["upload_ms"],
+        "device_ms": st["device_ms"],
+        "rounds": st["rounds"],
+        "passes": st["passes"],
+        "graph": bool(st["graph"]),
+        "autotune_s": tune_s,
+        "fidelity_vs_f64": {"max_infidelity": float(infid.max()), "median_infidelity": float(np.median(infid)),
+                            "max_abs": float(m.max()), "reference": "the same circuits in complex128 on the GPU"},
+    }
+
+
+# ----------------------------------------------------------------------- c4
+
+def bench_c4(args) -> dict:
+    import numpy as np
+
+    import everett_amd as ev
+
+    R = args.serve_requests
+    reqs = [ev.workloads.config4_request(r) for r in range(R)]
+    for n in sorted({c.num_qubits() for c, _ in reqs}):
+        if ev.tune_get(n, "c64") is None:
+            ev.autotune(n, "c64", 8)
+
+    # capacity under a burst of the first requests
+    burst = reqs[:min(48, R)]
+    with ev.Server(precision="c64") as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(c, shots=1024, seed=s) for c, s in burst]
+        for i in ids:
+            srv.wait(i)
+        cap = len(burst) / (time.perf_counter() - t0)
+    rate = args.serve_load * cap
+    arrivals = ev.workloads.poisson_arrivals(R, rate)
+
+    with ev.Server(precision="c64") as srv:
+        ids = [None] * R
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, ((c, s), at) in enumerate(zip(reqs, arrivals)):
+                d = at - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = srv.submit(c, shots=1024, seed=s)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        infos = []
+        for k in range(R):
+            _, info = srv.wait(ids[k])
+            infos.append(info)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([i["done_ms"] - i["submit_ms"] for i in infos])
+    span_s = (max(i["done_ms"] for i in infos) - min(i["submit_ms"] for i in infos)) * 1e-3
+    served = R / span_s
+
+    # single-request plain-GPU baseline: the same requests one at a time, one launch per gate
+    t1 = time.perf_counter()
+    done = 0
+    for c, s in reqs:
+        ex = ev.StateVectorBackend.run(c, precision="c64", plain=True)
+        ex.state().sample(1024, s)
+        done += 1
+        if time.perf_counter() - t1 > args.serve_plain_seconds:
+            break
+    plain_rate = done / (time.perf_counter() - t1)
+
+    # c64 fidelity against complex128 for a sample of the served circuits
+    sample = [c for c, _ in reqs if c.num_qubits() <= 22][:64]
+    f, m = ev.Batch.run(sample, precision="c64").compare(ev.Batch.run(sample, precision="c128"))
+    return {
+        "workload": f"{R} requests, n = round(exp(U[ln 12, ln 28])), depth 20..200 brick-wall, 1024 shots each, "
+                    f"Poisson arrivals at {rate:.2f}/s ({args.serve_load:.0%} of the burst capacity), complex64",
+        "circuits_per_s": served,
+        "p50_latency_ms": float(np.percentile(lat, 50)),
+        "p99_latency_ms": float(np.percentile(lat, 99)),
+        "offered_rate_per_s": rate,
+        "burst_capacity_per_s": cap,
+        "single_request_plain_circuits_per_s": plain_rate,
+        "single_request_plain_sample": f"first {done} requests in arrival order, one at a time, one launch per gate",
+        "speedup_vs_single_plain": served / plain_rate,
+        "steps": st["steps"],
+        "busy_ms": st["busy_ms"],
+        "bytes_peak": st["bytes_peak"],
+        "wall_s": wall,
+        "c64_vs_c128_sample": {"circuits": len(sample), "max_infidelity": float((1 - f).max()),
+                               "max_abs": float(m.max())},
+    }
+
+
+# --------------------------------------------------------------------- main
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
-    ap.add_argument("--n", type=int, default=None, help="override the qubit count (profiling runs)")
+    ap.add_argument("--workload", default="all", choices=["all", "c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--n", type=int, default=None, help="override the c1 qubit count (profiling runs)")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     ap.add_argument("--plain", action="store_true", help="one launch per gate instead of fused passes")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--serve-plain-seconds", type=float, default=20.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,98 +334,69 @@ def main() -> int:
 
     steps = args.steps if args.steps is not None else (1 if args.plain else 5)
     warmup = args.warmup if args.warmup is not None else 1
+    wl = args.workload
+    n1 = args.n or 30
 
-    circuit, n, desc = workload(args.workload, args.n)
-    gates = len(circuit)
-    print(f"[bench] workload {desc}: {gates} gates, precision {args.precision}, "
-          f"{'plain' if args.plain else 'fused'} path", file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "c128" else "f32",
+           "data": "synthetic (seeded random circuits; no datasets)"}
 
-    t = time.perf_counter()
-    state = ev.State.zero(n, args.precision)
-    alloc_ms = (time.perf_counter() - t) * 1e3
-
-    for w in range(warmup):
-        ev.run_into(state, circuit, 0, plain=args.plain)
-        print(f"[bench] warmup {w + 1}/{warmup} done", file=sys.stderr, flush=True)
-
-    # Timed region: K steps, HIP events around every kernel launch.
-    state.norm_sqr()  # synchronises the handle's stream
-    t0 = time.perf_counter()
-    agg: dict = {}
-    dev_ms = 0.0
-    for k in range(steps):
-        ev.run_into(state, circuit, 0, plain=args.plain, profile=True)
-        dev_ms += ev.last_run_stats().device_ms
-        for ks in ev.last_kernel_stats():
-            a = agg.setdefault(ks["name"], {"launches": 0, "total_ms": 0.0, "bytes": 0.0})
-            a["launches"] += ks["launches"]
-            a["total_ms"] += ks["total_ms"]
-            a["bytes"] += ks["bytes"]
-        print(f"[bench] step {k + 1}/{steps} done", file=sys.stderr, flush=True)
-    norm = state.norm_sqr()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    if abs(norm - 1.0) > 1e-6:
-        print(f"[bench] norm drifted to {norm}", file=sys.stderr)
-        return 1
-
-    amp_bytes = 16 if args.precision == "c128" else 8
-    step_bytes = sum(a["bytes"] for a in agg.values()) / steps
-    dom_name, dom = max(agg.items(), key=lambda kv: kv[1]["total_ms"])
-    per_launch_bytes = dom["bytes"] / dom["launches"]
-    avg_ms = dom["total_ms"] / dom["launches"]
-    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tf = ROOT / "profiles" / "traffic.json"
-    if tf.exists():
-        entry = json.loads(tf.read_text()).get(f"{dom_name}:{args.workload}:{n}:{args.precision}")
-        if entry:
-            traffic = entry.get("hbm_bytes_per_launch")
-
-    out = {
-        "metric": METRIC,
-        "value": gates * steps / wall,
-        "unit": "gates/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
-        "ms_per_step": wall / steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "c128" else "f32",
-        "data": "synthetic (seeded random brick-wall circuit + QFT; no datasets)",
-        "config": {
-            "workload": desc,
-            "n_qubits": n,
-            "gates": gates,
-            "precision": args.precision,
-            "path": "plain" if args.plain else "fused",
-        },
-        "hbm_gbps": step_bytes / (wall / steps) / 1e9,
-        "device_ms_per_step": dev_ms / steps,
-        "state_alloc_ms": alloc_ms,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom_name,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": traffic,
-            "bytes_per_launch": per_launch_bytes,
-            "avg_launch_ms": avg_ms,
-            "launches_per_step": dom["launches"] / steps,
-        },
-        "kernels": {k: {"launches_per_step": v["launches"] / steps, "ms_per_step": v["total_ms"] / steps,
-                        "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["total_ms"] else None}
-                    for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["total_ms"])},
-        "cpu_baseline": None,
-    }
-    del state
-    if not args.no_cpu:
-        print(f"[bench] cpu baseline ({args.cpu_seconds:.0f} s sample)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(circuit, n, args.cpu_seconds)
+    copy_gbps = ev.copy_bandwidth(4 << 30, 10)
+    out["copy_peak_gbps"] = copy_gbps
+    log(f"streaming copy: {copy_gbps:.0f} GB/s")
+    if wl in ("all", "c1"):
+        c1 = ev.workloads.config1(n1, 40)
+        if not args.plain and not args.no_autotune:
+            t = time.perf_counter()
+            tuned = ev.autotune(n1, args.precision, 8)
+            log(f"autotune n={n1}: {tuned} ({time.perf_counter() - t:.1f} s)")
+        log(f"c1: {len(c1)} gates at n={n1}, {args.precision}, {'plain' if args.plain else 'fused'}")
+        r = bench_sv("c1", c1, n1, f"{n1}-qubit brick-wall depth 40 (seed {n1}) + QFT({n1})", args, steps, warmup)
+        out.update({
+            "value": r["gates_per_s"],
+            "unit": "gates/s",
+            "ms_per_step": r["ms_per_step"],
+            "config": {"workload": r["workload"], "n_qubits": n1, "gates": r["gates"], "precision": args.precision,
+                       "path": "plain" if args.plain else "fused", "tune": ev.tune_get(n1, args.precision)},
+            "hbm_gbps": r["hbm_gbps"],
+            "device_ms_per_step": r["device_ms_per_step"],
+            "passes_per_step": r["passes_per_step"],
+            "state_alloc_ms": r["state_alloc_ms"],
+            "roofline": dict(r["roofline"], frac_of_copy=r["roofline"]["achieved"] / copy_gbps),
+            "kernels": r["kernels"],
+            "cpu_baseline": None,
+        })
+        log(f"c1: {r['gates_per_s']:.0f} gates/s, {r['ms_per_step']:.1f} ms/step, roofline "
+            f"{r['roofline']['achieved']:.0f} GB/s")
+        if not args.no_cpu:
+            log(f"c1 cpu baseline ({args.cpu_seconds:.0f} s sample)")
+            out["cpu_baseline"] = cpu_baseline(c1, n1, args.cpu_seconds)
+    extra = {}
+    if wl in ("all", "c0"):
+        c0 = ev.workloads.config0(20, 20)
+        r = bench_sv("c0", c0, 20, "20-qubit brick-wall depth 20 (seed 20)", args, 10, 2)
+        extra["c0"] = {k: r[k] for k in ("workload", "gates", "gates_per_s", "ms_per_step", "device_ms_per_step",
+                                         "passes_per_step")}
+        log(f"c0: {r['gates_per_s']:.0f} gates/s")
+    if wl in ("all", "c2"):
+        extra["c2"] = bench_c2(args)
+        log(f"c2: {extra['c2']['gates_per_s']:.0f} gates/s")
+    if wl in ("all", "c3"):
+        extra["c3"] = bench_c3(args)
+        log(f"c3: {extra['c3']['circuits_per_s']:.0f} circuits/s")
+    if wl in ("all", "c4"):
+        extra["c4"] = bench_c4(args)
+        log(f"c4: {extra['c4']['circuits_per_s']:.1f} circuits/s, p50 {extra['c4']['p50_latency_ms']:.1f} ms, "
+            f"p99 {extra['c4']['p99_latency_ms']:.1f} ms")
+    if "value" not in out:  # a single non-headline workload was asked for
+        key = next(iter(extra))
+        e = extra[key]
+        out.update({"value": e.get("circuits_per_s", e.get("gates_per_s")),
+                    "unit": "circuits/s" if "circuits_per_s" in e else "gates/s",
+                    "ms_per_step": e.get("ms_per_step", e.get("wall_ms", e.get("ms"))),
+                    "config": {"workload": e["workload"]}, "roofline": None, "cpu_baseline": None})
+    if extra:
+        out["workloads"] = extra
     print(json.dumps(out), flush=True)
     return 0
 

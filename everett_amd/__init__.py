@@ -17,6 +17,7 @@ from .backend import (
     RunStats,
     StateVectorBackend,
     autotune,
+    copy_bandwidth,
     last_kernel_stats,
     last_run_stats,
     run_into,
@@ -45,7 +46,19 @@ from .state import State, measure_qubit
 __all__ = [
     "algorithms", "kernel", "workloads", "Batch", "Server", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
     "StateVectorBackend", "Execution", "DensityMatrixBackend", "DensityMatrix", "DensityMatrixExecution",
-    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "EverettError", "QubitOutOfRange",
+    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "copy_bandwidth", "EverettError", "QubitOutOfRange",
     "ClassicalBitOutOfRange", "DuplicateQubit", "DimensionMismatch", "InvalidArgument", "DeviceError",
     "OutOfMemory", "AdmissionError", "LimitError", "StateError",
 ]
+
+
+def _load_tuning() -> None:
+    from . import tune
+
+    try:
+        tune.load()
+    except FileNotFoundError:
+        pass
+
+
+_load_tuning()

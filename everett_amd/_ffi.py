@@ -204,6 +204,7 @@ PROTOTYPES = {
     "ev_batch_norms": (I, [P, P]),
     "ev_batch_compare": (I, [P, P, P, P]),
     "ev_batch_get_stats": (I, [P, C.POINTER(EvBatchStats)]),
+    "ev_copy_bandwidth": (I, [SZ, C.c_uint32, C.POINTER(D)]),
     "ev_autotune": (I, [U64, I, C.c_uint32, C.POINTER(EvTuneEntry)]),
     "ev_tune_get": (I, [U64, I, C.POINTER(EvTuneEntry), C.POINTER(I)]),
     "ev_tune_set": (I, [U64, I, C.POINTER(EvTuneEntry)]),

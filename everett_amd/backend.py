@@ -245,3 +245,10 @@ def tune_set(n: int, precision, entry: dict) -> None:
     e.max_gates_per_pass = entry["max_gates_per_pass"]
     e.ms_per_gate = entry.get("ms_per_gate", 0.0)
     check(_ffi.lib().ev_tune_set(n, precision_code(precision), C.byref(e)))
+
+
+def copy_bandwidth(nbytes: int = 4 << 30, reps: int = 10) -> float:
+    """Measured streaming-copy bandwidth (read + write GB/s) of this GPU."""
+    out = C.c_double()
+    check(_ffi.lib().ev_copy_bandwidth(nbytes, reps, C.byref(out)))
+    return out.value

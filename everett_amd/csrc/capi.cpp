@@ -856,6 +856,44 @@ int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e)
     });
 }
 
+int ev_copy_bandwidth(size_t bytes, uint32_t reps, double *gbps)
+{
+    return guard([&] {
+        NONNULL(gbps);
+        EV_REQUIRE(bytes >= 32 && bytes % 32 == 0 && reps > 0, EV_E_ARG, "bytes must be a positive multiple of 32");
+        void *a = nullptr, *b = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (st) (void)hipStreamSynchronize(st);
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        try {
+            EV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            EV_HIP(hipMalloc(&a, bytes));
+            EV_HIP(hipMalloc(&b, bytes));
+            EV_HIP(hipEventCreate(&e0));
+            EV_HIP(hipEventCreate(&e1));
+            ev::kern::copy(a, b, bytes, st);  // warm-up (also touches both buffers)
+            EV_HIP(hipEventRecord(e0, st));
+            for (uint32_t r = 0; r < reps; ++r) ev::kern::copy(a, b, bytes, st);
+            EV_HIP(hipEventRecord(e1, st));
+            EV_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            EV_HIP(hipEventElapsedTime(&ms, e0, e1));
+            *gbps = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, void *stages, void *ops,
                   ev_c128 *coef, ev_plan_info *info)
 {

@@ -43,13 +43,20 @@ enum OpKind : uint8_t {
 
 enum OpFlags : uint8_t { kD0IsOne = 1 };
 
+// Every field is a full dword so the kernel reads descriptors with scalar
+// (SMEM) loads; code = kind | b0 << 8 | b1 << 16 | flags << 24.
 struct DevOp {
-    uint8_t kind, b0, b1, flags;
+    uint32_t code;
     uint32_t creg;   // condition on the register index j: (j & creg) == creg
     uint64_t cglob;  // condition on the thread's fixed global index bits
     uint32_t coef;   // offset (complex entries) into the coefficient pool
     uint32_t tq;     // kDiagT target qubit
 };
+
+EV_HOST_DEVICE inline uint32_t op_code(uint32_t kind, uint32_t b0, uint32_t b1, uint32_t flags)
+{
+    return kind | (b0 << 8) | (b1 << 16) | (flags << 24);
+}
 
 struct Stage {
     int32_t op_begin, op_end;

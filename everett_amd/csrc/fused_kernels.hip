@@ -134,7 +134,7 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
 }
 
 #define EV_CASE_B(fn, ...)                  \
-    switch (op.b0) {                        \
+    switch (b0) {                           \
     case 0: fn<V, 0>(__VA_ARGS__); break;   \
     case 1: fn<V, 1>(__VA_ARGS__); break;   \
     case 2: fn<V, 2>(__VA_ARGS__); break;   \
@@ -142,7 +142,7 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
     }
 
 #define EV_CASE_PAIR(fn, ...)                        \
-    switch (op.b0 * 4 + op.b1) {                     \
+    switch (b0 * 4 + b1) {                           \
     case 1 * 4 + 0: fn<V, 1, 0>(__VA_ARGS__); break; \
     case 2 * 4 + 0: fn<V, 2, 0>(__VA_ARGS__); break; \
     case 2 * 4 + 1: fn<V, 2, 1>(__VA_ARGS__); break; \
@@ -156,7 +156,9 @@ __device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfi
 {
     const V *m = coef + op.coef;
     const bool tok = (gfix & op.cglob) == op.cglob;
-    switch (op.kind) {
+    const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
+    const uint32_t flags = op.code >> 24;
+    switch (kind) {
     case kU1: {
         const V m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
         EV_CASE_B(op_u1, v, m0, m1, m2, m3);
@@ -170,7 +172,7 @@ __device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfi
     case kCX: EV_CASE_B(op_cx, v, op.creg, tok); break;
     case kDiagR: {
         const V d0 = m[0], d1 = m[1];
-        const bool one = (op.flags & kD0IsOne) != 0;
+        const bool one = (flags & kD0IsOne) != 0;
         EV_CASE_B(op_diagr, v, d0, d1, one, op.creg, tok);
         break;
     }

@@ -44,6 +44,9 @@ void collapse(void *psi, Prec p, int n, int k, int outcome, double scale, hipStr
 // src/state.rs:103-111: multiply every amplitude by s.
 void scale(void *psi, Prec p, int n, double s, hipStream_t st);
 
+// Streaming copy of `bytes` (a multiple of 32), the HBM bandwidth reference.
+void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+
 // Precision conversion between a c128 staging buffer and c64 storage.
 void c128_to_c64(const void *src, void *dst, uint64_t len, hipStream_t st);
 void c64_to_c128(const void *src, void *dst, uint64_t len, hipStream_t st);

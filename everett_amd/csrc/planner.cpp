@@ -289,6 +289,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
             for (int id : stake) {
                 const GOp &op = g[id];
                 DevOp d{};
+                uint32_t kind = 0, b0 = 0, b1 = 0, flags = 0;
                 for (uint64_t m = op.cmask; m; m &= m - 1) {
                     const int q = __builtin_ctzll(m);
                     const int rb = regbit(q);
@@ -299,40 +300,40 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
                 switch (op.kind) {
                 case G_U1:
                 case G_CU1:
-                    d.kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
-                    d.b0 = uint8_t(regbit(op.q0));
+                    kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
+                    b0 = uint32_t(regbit(op.q0));
                     P.coef.insert(P.coef.end(), pool.begin() + op.coef, pool.begin() + op.coef + 4);
                     break;
                 case G_CX:
-                    d.kind = fused::kCX;
-                    d.b0 = uint8_t(regbit(op.q0));
+                    kind = fused::kCX;
+                    b0 = uint32_t(regbit(op.q0));
                     break;
                 case G_DIAG: {
                     const c128 d0 = pool[op.coef], d1 = pool[op.coef + 1];
                     const int rb = regbit(op.q0);
                     if (rb >= 0) {
-                        d.kind = fused::kDiagR;
-                        d.b0 = uint8_t(rb);
+                        kind = fused::kDiagR;
+                        b0 = uint32_t(rb);
                     } else {
-                        d.kind = fused::kDiagT;
+                        kind = fused::kDiagT;
                         d.tq = uint32_t(op.q0);
                     }
-                    if (d0.re == 1 && d0.im == 0) d.flags |= fused::kD0IsOne;
+                    if (d0.re == 1 && d0.im == 0) flags |= fused::kD0IsOne;
                     P.coef.push_back(d0);
                     P.coef.push_back(d1);
                     break;
                 }
                 case G_U2: {
-                    d.kind = fused::kU2;
+                    kind = fused::kU2;
                     const int ra = regbit(op.q0), rb = regbit(op.q1);
                     const c128 *m = pool.data() + op.coef;
                     if (ra > rb) {
-                        d.b0 = uint8_t(ra);
-                        d.b1 = uint8_t(rb);
+                        b0 = uint32_t(ra);
+                        b1 = uint32_t(rb);
                         P.coef.insert(P.coef.end(), m, m + 16);
                     } else {  // reorder the basis |q_a q_b> -> |q_b q_a>
-                        d.b0 = uint8_t(rb);
-                        d.b1 = uint8_t(ra);
+                        b0 = uint32_t(rb);
+                        b1 = uint32_t(ra);
                         auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
                         c128 mm[16];
                         for (int r = 0; r < 4; ++r)
@@ -342,13 +343,14 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
                     break;
                 }
                 case G_SWAP: {
-                    d.kind = fused::kSwap;
+                    kind = fused::kSwap;
                     const int ra = regbit(op.q0), rb = regbit(op.q1);
-                    d.b0 = uint8_t(std::max(ra, rb));
-                    d.b1 = uint8_t(std::min(ra, rb));
+                    b0 = uint32_t(std::max(ra, rb));
+                    b1 = uint32_t(std::min(ra, rb));
                     break;
                 }
                 }
+                d.code = fused::op_code(kind, b0, b1, flags);
                 P.devops.push_back(d);
             }
             st.op_end = int(P.devops.size());

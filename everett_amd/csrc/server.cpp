@@ -280,7 +280,8 @@ void scheduler(ev_server *s)
         for (auto &r : admitted) {
             try {
                 admit(s, *r);
-                s->active.push_back(r);
+                if (r->plan.passes.empty()) finish(s, r);  // no gates: sample |0..0> right away
+                else s->active.push_back(r);
             } catch (const Error &e) {
                 release(s, *r);
                 fail(s, r, e.code);

@@ -104,9 +104,11 @@ void sv_bloch(ev_sv *s, int q, double out[4])
     for (int i = 0; i < 4; ++i) out[i] = s->h_red[i];
 }
 
-// src/measure.rs:15-51
+// src/measure.rs:15-51.  A c64 state is first rescaled to unit f64 norm, so
+// the Born probabilities do not inherit the f32 drift of the gates before.
 bool sv_measure(ev_sv *s, int q, double u)
 {
+    if (s->prec == kern::C64) sv_normalize(s);
     const double p1 = sv_p1(s, q);
     const bool outcome = u < p1;
     const double surviving = outcome ? p1 : 1.0 - p1;
@@ -385,6 +387,7 @@ void sv_execute(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const 
     } else {
         execute_planned(s, c, rng, classical, opt, st, prof);
     }
+    if (s->prec == kern::C64) sv_normalize(s);  // f64 renormalisation of the f32 bulk state
     st.device_ms = timer.stop();
     st.kernels = prof.collect();
     last_stats() = st;

@@ -413,7 +413,20 @@ inline dim3 rgrid(uint64_t work)
     return dim3(static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(b, kReduceBlocks))));
 }
 
+// Plain streaming copy (16-byte vectors), the bandwidth reference.
+__global__ void k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, uint64_t n4)
+{
+    GRID_STRIDE(i, n4) dst[i] = src[i];
+}
+
 }  // namespace
+
+void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t n4 = bytes / sizeof(double4);
+    hipLaunchKernelGGL(k_copy, grid_for(n4), dim3(kBlock), 0, st, static_cast<const double4 *>(src),
+                       static_cast<double4 *>(dst), n4);
+}
 
 void init_basis(void *psi, Prec p, int n, uint64_t basis, hipStream_t st)
 {

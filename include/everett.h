@@ -318,6 +318,12 @@ int ev_autotune(uint64_t n, ev_precision p, uint32_t depth, ev_tune_entry *out);
 int ev_tune_get(uint64_t n, ev_precision p, ev_tune_entry *out, int *found);
 int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e);
 
+/* Measured HBM copy bandwidth: a plain read-once/write-once kernel over two
+ * device buffers of `bytes` each, `reps` launches timed with HIP events.
+ * Reports read + write bytes per second (the achievable-peak reference for
+ * the roofline). */
+int ev_copy_bandwidth(size_t bytes, uint32_t reps, double *gbps);
+
 /* ---------------------------------------------------------- introspection */
 
 /* The fused pass plan ev_sv_run would execute for a measurement-free circuit

@@ -20,8 +20,12 @@ PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"),
 STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("pad", "<i4"),
                         ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "i1", (13,)), ("gpos", "i1", (13,)),
                         ("pad2", "i1", (6,))])
-OP_DTYPE = np.dtype([("kind", "u1"), ("b0", "u1"), ("b1", "u1"), ("flags", "u1"), ("creg", "<u4"),
-                     ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4")])
+OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4")])
+
+
+def decode(o):
+    c = int(o["code"])
+    return c & 0xFF, (c >> 8) & 0xFF, (c >> 16) & 0xFF, c >> 24
 
 U1, CU1, CX, DIAGR, DIAGT, U2, SWAP = range(7)
 
@@ -80,10 +84,10 @@ def check_plan(n, passes, stages, ops, coef):
                 assert gj and gj & (gj - 1) == 0
                 regq.add(gj.bit_length() - 1)
             for o in ops[st["op_begin"]: st["op_end"]]:
-                k = int(o["kind"])
-                assert k <= SWAP and o["b0"] < 4 and o["b1"] < 4 and o["creg"] < 16
+                k, b0, b1, _ = decode(o)
+                assert k <= SWAP and b0 < 4 and b1 < 4 and o["creg"] < 16
                 if k in (U2, SWAP):
-                    assert o["b0"] > o["b1"]
+                    assert b0 > b1
                 assert o["coef"] + _coef_span(k) <= len(coef)
                 cg = int(o["cglob"])
                 assert cg < (1 << n) and all(not (cg >> q) & 1 for q in regq)
@@ -92,7 +96,7 @@ def check_plan(n, passes, stages, ops, coef):
 
 
 def _apply(o, v, gfix, coef):
-    k, B, BL = int(o["kind"]), int(o["b0"]), int(o["b1"])
+    k, B, BL, flags = decode(o)
     creg, cglob = int(o["creg"]), np.uint64(o["cglob"])
     tok = (gfix & cglob) == cglob
     m = coef[int(o["coef"]):]
@@ -117,7 +121,7 @@ def _apply(o, v, gfix, coef):
             v[:, j] = np.where(tok, a1, a0)
             v[:, j1] = np.where(tok, a0, a1)
     elif k == DIAGR:
-        one = bool(o["flags"] & 1)
+        one = bool(flags & 1)
         for j in range(16):
             if not condj[j]:
                 continue

@@ -64,8 +64,10 @@ def test_fused_with_measurement_segments():
 def test_fused_c64_fidelity(n):
     c = ev.workloads.config1(n, 20)
     want, _ = oracle.sv_run(c.ops(), n)
-    got = ev.StateVectorBackend.run(c, precision="c64").state().amplitudes()
-    fid = abs(np.vdot(want, got)) ** 2
+    s = ev.StateVectorBackend.run(c, precision="c64").state()
+    assert abs(s.norm_sqr() - 1.0) < 1e-6  # renormalised in f64 at the end of the run
+    got = s.amplitudes()
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
     assert np.max(np.abs(got - want)) < 1e-5 and 1 - fid < 1e-6
 
 

@@ -80,3 +80,10 @@ def test_admission_by_memory_and_concurrent_submitters():
     with ev.Server(precision="c64", memory_bytes=budget) as srv:
         with pytest.raises(ev.AdmissionError):
             srv.submit(big, shots=1)
+
+
+def test_empty_circuit_request_samples_zero_state():
+    with ev.Server(precision="c128") as srv:
+        rid = srv.submit(ev.Circuit(9), shots=8, seed=1)
+        samples, info = srv.wait(rid)
+    assert info["status"] == 0 and list(samples) == [0] * 8
