@@ -159,6 +159,8 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             std::memcpy(host.data() + loc[i].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
             if (!P.devops.empty())
                 std::memcpy(host.data() + loc[i].op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
+            if (c64)
+                fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host.data() + loc[i].op_off), P.devops.size());
             if (!c64) {
                 std::memcpy(host.data() + loc[i].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
             } else {

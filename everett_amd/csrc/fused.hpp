@@ -15,6 +15,7 @@
 // stages the tile goes back to LDS and is re-read with the next mapping.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #if defined(__HIPCC__)
@@ -44,29 +45,35 @@ enum OpKind : uint8_t {
 enum OpFlags : uint8_t { kD0IsOne = 1 };
 
 // Every field is a full dword so the kernel reads descriptors with scalar
-// (SMEM) loads; code = kind | b0 << 8 | b1 << 16 | flags << 24.
+// (SMEM) loads; code = kind | b0 << 8 | b1 << 16 | flags << 24.  One-qubit
+// and diagonal ops carry their coefficients inline (one fetch per op); kU2
+// indexes the pass's coefficient pool.  In c64 runs the inline coefficients
+// are stored as floats in the first half of m.
 struct DevOp {
     uint32_t code;
     uint32_t creg;   // condition on the register index j: (j & creg) == creg
     uint64_t cglob;  // condition on the thread's fixed global index bits
-    uint32_t coef;   // offset (complex entries) into the coefficient pool
+    uint32_t coef;   // kU2: offset (complex entries) into the coefficient pool
     uint32_t tq;     // kDiagT target qubit
+    uint32_t pad[2];
+    double m[8];     // kU1/kCU1: m00..m11 {re, im}; kDiagR/kDiagT: d0, d1
 };
+static_assert(sizeof(DevOp) == 96, "DevOp is fetched as 96 bytes");
 
 EV_HOST_DEVICE inline uint32_t op_code(uint32_t kind, uint32_t b0, uint32_t b1, uint32_t flags)
 {
     return kind | (b0 << 8) | (b1 << 16) | (flags << 24);
 }
 
+// All fields are dwords (or wider) so the kernel reads them with scalar loads.
 struct Stage {
     int32_t op_begin, op_end;
     int32_t np;               // thread-id bits (t - 4)
     int32_t pad;
     uint32_t roff[kRegs];     // swizzled LDS slot offset of register j
     uint64_t goff[kRegs];     // global index offset of register j
-    int8_t p[13];             // thread-id bit i -> local position p[i]
-    int8_t gpos[13];          // thread-id bit i -> global qubit
-    int8_t pad2[6];
+    int32_t p[kMaxTile];      // thread-id bit i -> local position p[i]
+    int32_t gpos[kMaxTile];   // thread-id bit i -> global qubit
 };
 
 struct Pass {
@@ -74,9 +81,20 @@ struct Pass {
     int32_t nstages;
     int32_t stage_begin;
     int32_t ncomp;            // n - t
-    int8_t q[16];             // local bit -> global qubit (ascending)
-    int8_t comp[40];          // tile-index bit -> global qubit (the non-tile qubits, ascending)
+    int32_t q[16];            // local bit -> global qubit (ascending)
+    int32_t comp[40];         // tile-index bit -> global qubit (the non-tile qubits, ascending)
 };
+
+// Rewrites the inline coefficients of ops for a c64 run (floats in the
+// first half of m).
+inline void inline_coef_to_c64(DevOp *ops, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        float f[8];
+        for (int k = 0; k < 8; ++k) f[k] = float(ops[i].m[k]);
+        for (int k = 0; k < 8; ++k) reinterpret_cast<float *>(ops[i].m)[k] = f[k];
+    }
+}
 
 // LDS slot of local index l: the low 4 bits are XORed with the higher
 // nibbles, so 16 consecutive lanes land in 16 distinct 16-byte slots of a

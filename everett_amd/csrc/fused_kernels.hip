@@ -154,10 +154,10 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
 template <class V>
 __device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfix, const V *__restrict__ coef)
 {
-    const V *m = coef + op.coef;
-    const bool tok = (gfix & op.cglob) == op.cglob;
     const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
     const uint32_t flags = op.code >> 24;
+    const V *m = (kind == kU2) ? coef + op.coef : reinterpret_cast<const V *>(op.m);
+    const bool tok = (gfix & op.cglob) == op.cglob;
     switch (kind) {
     case kU1: {
         const V m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];

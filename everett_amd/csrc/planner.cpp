@@ -223,8 +223,8 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
         ps.ncomp = n - t;
         int k = 0, c = 0;
         for (int q = 0; q < n; ++q) {
-            if ((Q >> q) & 1) ps.q[k++] = int8_t(q);
-            else ps.comp[c++] = int8_t(q);
+            if ((Q >> q) & 1) ps.q[k++] = q;
+            else ps.comp[c++] = q;
         }
         ps.stage_begin = int(P.stages.size());
         auto local = [&](int q) { return rank_in(Q, q); };
@@ -263,7 +263,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
             for (int l = 0; l < t; ++l) {
                 if ((S >> l) & 1) sp[ns++] = l;
                 else {
-                    st.p[np] = int8_t(l);
+                    st.p[np] = l;
                     st.gpos[np] = ps.q[l];
                     ++np;
                 }
@@ -302,7 +302,10 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
                 case G_CU1:
                     kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
                     b0 = uint32_t(regbit(op.q0));
-                    P.coef.insert(P.coef.end(), pool.begin() + op.coef, pool.begin() + op.coef + 4);
+                    for (int e = 0; e < 4; ++e) {
+                        d.m[2 * e] = pool[op.coef + e].re;
+                        d.m[2 * e + 1] = pool[op.coef + e].im;
+                    }
                     break;
                 case G_CX:
                     kind = fused::kCX;
@@ -319,8 +322,10 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
                         d.tq = uint32_t(op.q0);
                     }
                     if (d0.re == 1 && d0.im == 0) flags |= fused::kD0IsOne;
-                    P.coef.push_back(d0);
-                    P.coef.push_back(d1);
+                    d.m[0] = d0.re;
+                    d.m[1] = d0.im;
+                    d.m[2] = d1.re;
+                    d.m[3] = d1.im;
                     break;
                 }
                 case G_U2: {
@@ -407,6 +412,7 @@ void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
     std::memcpy(host.data() + o_pass, P.passes.data(), P.passes.size() * sizeof(Pass));
     std::memcpy(host.data() + o_stage, P.stages.data(), P.stages.size() * sizeof(Stage));
     if (!P.devops.empty()) std::memcpy(host.data() + o_ops, P.devops.data(), P.devops.size() * sizeof(DevOp));
+    if (c64) fused::inline_coef_to_c64(reinterpret_cast<DevOp *>(host.data() + o_ops), P.devops.size());
     if (!c64) {
         std::memcpy(host.data() + o_coef, P.coef.data(), P.coef.size() * sizeof(c128));
     } else {

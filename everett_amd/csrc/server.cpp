@@ -155,6 +155,7 @@ void admit(ev_server *s, Request &r)
     std::memcpy(host.data() + r.pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
     std::memcpy(host.data() + r.stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
     if (!P.devops.empty()) std::memcpy(host.data() + r.op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
+    if (s->c64) fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host.data() + r.op_off), P.devops.size());
     if (!s->c64) {
         std::memcpy(host.data() + r.coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
     } else {

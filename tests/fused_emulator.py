@@ -16,11 +16,11 @@ from everett_amd import _ffi
 from everett_amd.errors import check
 
 PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"), ("ncomp", "<i4"),
-                       ("q", "i1", (16,)), ("comp", "i1", (40,))])
+                       ("q", "<i4", (16,)), ("comp", "<i4", (40,))])
 STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("pad", "<i4"),
-                        ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "i1", (13,)), ("gpos", "i1", (13,)),
-                        ("pad2", "i1", (6,))])
-OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4")])
+                        ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,))])
+OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4"),
+                     ("pad", "<u4", (2,)), ("m", "<c16", (4,))])
 
 
 def decode(o):
@@ -53,7 +53,7 @@ def swz(l):
 
 
 def _coef_span(kind):
-    return {U1: 4, CU1: 4, DIAGR: 2, DIAGT: 2, U2: 16}.get(kind, 0)
+    return {U2: 16}.get(kind, 0)
 
 
 def check_plan(n, passes, stages, ops, coef):
@@ -99,7 +99,7 @@ def _apply(o, v, gfix, coef):
     k, B, BL, flags = decode(o)
     creg, cglob = int(o["creg"]), np.uint64(o["cglob"])
     tok = (gfix & cglob) == cglob
-    m = coef[int(o["coef"]):]
+    m = coef[int(o["coef"]):] if k == U2 else o["m"]
     condj = [((j & creg) == creg) for j in range(16)]
     if k in (U1, CU1):
         for j in range(16):

@@ -200,17 +200,17 @@ def test_workload_sizes():
 def test_tune_table_roundtrip_and_plan_uses_it():
     from tests.fused_emulator import get_plan
 
-    assert ev.tune_get(17, "c64") is None
-    ev.tune_set(17, "c64", {"tile_qubits": 9, "low_qubits": 3, "max_gates_per_pass": 7})
-    e = ev.tune_get(17, "c64")
+    assert ev.tune_get(31, "c64") is None  # the shipped table covers 8..30
+    ev.tune_set(31, "c64", {"tile_qubits": 9, "low_qubits": 3, "max_gates_per_pass": 7})
+    e = ev.tune_get(31, "c64")
     assert (e["tile_qubits"], e["low_qubits"], e["max_gates_per_pass"]) == (9, 3, 7)
-    c = ev.workloads.config0(17, 3)
+    c = ev.workloads.config0(31, 3)
     _, _, _, _, t = get_plan(c, precision="c64")
     assert t == 9
     _, _, _, _, t = get_plan(c, precision="c64", tile_qubits=11)  # explicit options win
     assert t == 11
     with pytest.raises(ev.InvalidArgument):
-        ev.tune_set(17, "c64", {"tile_qubits": 13, "low_qubits": 3, "max_gates_per_pass": 7})
+        ev.tune_set(31, "c64", {"tile_qubits": 13, "low_qubits": 3, "max_gates_per_pass": 7})
 
 
 @pytest.mark.parametrize("low", [1, 2, 3, 5, 6])
