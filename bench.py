@@ -315,7 +315,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=None, help="override the c1 qubit count (profiling runs)")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     ap.add_argument("--plain", action="store_true", help="one launch per gate instead of fused passes")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="never tune; sizes without a table entry use defaults")
+    ap.add_argument("--retune", action="store_true", help="tune the c1 size even when the table has an entry")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -346,7 +347,7 @@ def main() -> int:
     log(f"streaming copy: {copy_gbps:.0f} GB/s")
     if wl in ("all", "c1"):
         c1 = ev.workloads.config1(n1, 40)
-        if not args.plain and not args.no_autotune:
+        if not args.plain and not args.no_autotune and (args.retune or ev.tune_get(n1, args.precision) is None):
             t = time.perf_counter()
             tuned = ev.autotune(n1, args.precision, 8)
             log(f"autotune n={n1}: {tuned} ({time.perf_counter() - t:.1f} s)")

@@ -225,7 +225,7 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(
-                f"everett_amd native library missing at {LIB_PATH}; build it with `python -m everett_amd.build`"
+                f"everett_amd native library missing at {LIB_PATH}; build it with `python everett_amd/build.py`"
             )
         h = C.CDLL(str(LIB_PATH))
         for name, (res, args) in PROTOTYPES.items():

@@ -2,7 +2,8 @@
 
 HIP sources (*.hip) go through hipcc for gfx950; host-only sources (*.cpp)
 through g++ against the HIP runtime headers.  Objects land in
-everett_amd/_build/.  Run as `python -m everett_amd.build`.
+everett_amd/_build/.  Run as `python everett_amd/build.py` (a script:
+importing the package would load the library it builds).
 """
 from __future__ import annotations
 

@@ -208,10 +208,7 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
         const fused::BatchEntry *dent = reinterpret_cast<const fused::BatchEntry *>(dbase + entries_off);
         auto enqueue = [&] {
             for (uint32_t r = 0; r < max_passes; ++r) {
-                const int t = round_t[r];
-                const unsigned per_cu = unsigned(std::max<size_t>(1, (160u << 10) / ((size_t(1) << t) * ab)));
-                const unsigned grid = 256u * std::min(per_cu, 8u);
-                fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], t, grid, b->stream);
+                fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], round_t[r], b->stream);
             }
         };
         hipEvent_t e0, e1;

@@ -1,17 +1,17 @@
 // The fused statevector pass kernel for gfx950 (descriptors in fused.hpp).
 //
-// One workgroup per tile of 2^t amplitudes (W = 2^(t-4) threads, 16
-// amplitudes per thread).  Rows of the tile are contiguous in HBM because the
-// lowest qubits are always tile qubits; the load puts consecutive lanes on
-// consecutive amplitudes.  The tile lives in LDS (64 KiB at most, XOR-swizzled
-// so every stage mapping reads conflict-free); each stage pulls 16 amplitudes
-// per thread into registers, applies its ops with compile-time register
-// indices (no scratch), and writes them back.  Matrix-vector products are
-// chained fused multiply-adds (one rounding per step); results agree with the
-// reference's per-product rounding to ~1e-15 relative per gate.
+// A tile is 2^t amplitudes handled by one workgroup of W = 2^(t-4) threads,
+// 16 amplitudes per thread.  A tile arrives straight in its first stage's
+// register mapping and leaves straight from its last stage's; only the
+// exchanges between stages go through LDS (64 KiB at most, XOR-swizzled so
+// the stage mappings read and write conflict-free).  Ops act on 16 registers
+// with compile-time register indices (no scratch).  Matrix-vector products
+// are chained fused multiply-adds (one rounding per step); results agree with
+// the reference's per-product rounding to ~1e-15 relative per gate.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -24,50 +24,84 @@ namespace {
 
 template <class V> using Regs = V[kRegs];
 
-// m0 * a0 + m1 * a1 as one fused chain per component (1 mul + 3 fma), which
-// rounds once per step instead of rounding each product and then the sum.
-template <class V> __device__ __forceinline__ V cmac2(V m0, V a0, V m1, V a1)
+// Op bodies.  Each output component is one fused multiply-add chain (one
+// rounding per step) whose last step reads the input component it replaces.
+
+// (a0, a1) <- (m0 a0 + m1 a1, m2 a0 + m3 a1).  Per output component one
+// chain: a0.re = m0.re a0.re + (-m0.im a0.im + (m1.re a1.re + -(m1.im a1.im)))
+// and likewise, the term of the replaced input last.
+template <bool MS, class V> __device__ __forceinline__ void mix2(V &a0, V &a1, V m0, V m1, V m2, V m3)
 {
-    V r;
-    r.x = dfma(m0.x, a0.x, dfma(-m0.y, a0.y, dfma(m1.x, a1.x, -(m1.y * a1.y))));
-    r.y = dfma(m0.x, a0.y, dfma(m0.y, a0.x, dfma(m1.x, a1.y, m1.y * a1.x)));
-    return r;
+    const auto p0x = dfma(-m0.y, a0.y, dfma(m1.x, a1.x, -(m1.y * a1.y)));
+    const auto p0y = dfma(m0.y, a0.x, dfma(m1.x, a1.y, m1.y * a1.x));
+    const auto p1x = dfma(m2.x, a0.x, dfma(-m2.y, a0.y, -(m3.y * a1.y)));
+    const auto p1y = dfma(m2.x, a0.y, dfma(m2.y, a0.x, m3.y * a1.x));
+    a0 = mk<V>(dfma(m0.x, a0.x, p0x), dfma(m0.x, a0.y, p0y));
+    a1 = mk<V>(dfma(m3.x, a1.x, p1x), dfma(m3.x, a1.y, p1y));
 }
 
-// sum_c m[c] * a[c] over four terms, same chained form.
-template <class V> __device__ __forceinline__ V cmac4(const V *m, V a0, V a1, V a2, V a3)
+// a <- d a with the reference's complex product (src/complex.rs:115-125):
+// (fma(d.re, a.re, -(d.im a.im)), fma(d.re, a.im, d.im a.re)).
+template <bool MS, class V> __device__ __forceinline__ void scale_by(V &a, V d)
 {
-    V r;
-    r.x = dfma(m[0].x, a0.x, dfma(-m[0].y, a0.y, dfma(m[1].x, a1.x, dfma(-m[1].y, a1.y,
-          dfma(m[2].x, a2.x, dfma(-m[2].y, a2.y, dfma(m[3].x, a3.x, -(m[3].y * a3.y))))))));
-    r.y = dfma(m[0].x, a0.y, dfma(m[0].y, a0.x, dfma(m[1].x, a1.y, dfma(m[1].y, a1.x,
-          dfma(m[2].x, a2.y, dfma(m[2].y, a2.x, dfma(m[3].x, a3.y, m[3].y * a3.x)))))));
-    return r;
+    a = cmul(d, a);
 }
 
-template <class V, int B> __device__ __forceinline__ void op_u1(Regs<V> &v, V m0, V m1, V m2, V m3)
+// a_r <- sum_c m[4r + c] a_c for a group of four registers.
+template <class V> __device__ __forceinline__ void mix4(V &a0, V &a1, V &a2, V &a3, const V *m)
 {
+    const V x[4] = {a0, a1, a2, a3};
+    decltype(V::x) px[4], py[4];
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) {
-        if (j & (1 << B)) continue;
-        const int j1 = j | (1 << B);
-        const V a0 = v[j], a1 = v[j1];
-        v[j] = cmac2(m0, a0, m1, a1);
-        v[j1] = cmac2(m2, a0, m3, a1);
+    for (int r = 0; r < 4; ++r) {
+        const V *row = m + 4 * r;
+        decltype(V::x) ax = -(row[r].y * x[r].y), ay = row[r].y * x[r].x;
+#pragma unroll
+        for (int c = 3; c >= 0; --c) {
+            if (c == r) continue;
+            ax = dfma(row[c].x, x[c].x, dfma(-row[c].y, x[c].y, ax));
+            ay = dfma(row[c].x, x[c].y, dfma(row[c].y, x[c].x, ay));
+        }
+        px[r] = ax;
+        py[r] = ay;
     }
+    a0 = mk<V>(dfma(m[0].x, x[0].x, px[0]), dfma(m[0].x, x[0].y, py[0]));
+    a1 = mk<V>(dfma(m[5].x, x[1].x, px[1]), dfma(m[5].x, x[1].y, py[1]));
+    a2 = mk<V>(dfma(m[10].x, x[2].x, px[2]), dfma(m[10].x, x[2].y, py[2]));
+    a3 = mk<V>(dfma(m[15].x, x[3].x, px[3]), dfma(m[15].x, x[3].y, py[3]));
 }
 
-template <class V, int B>
-__device__ __forceinline__ void op_cu1(Regs<V> &v, V m0, V m1, V m2, V m3, uint32_t creg, bool tok)
+// Workgroup barrier for the LDS exchanges between stages.  It orders LDS
+// only: __syncthreads() would also wait for this thread's outstanding global
+// stores and loads (a full workgroup fence), which would serialise a tile's
+// HBM write-back and the next tile's gather behind every barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-component value select (a select of the two values, never of their
+// addresses, so register arrays stay in registers).
+template <class V> __device__ __forceinline__ V pick(bool c, V a, V b)
+{
+    return mk<V>(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// Identity where the lane's global condition fails: conditional ops then run
+// unconditionally (an exact multiply by 1), with no per-register selects.
+template <class V> __device__ __forceinline__ V one_unless(bool tok, V m) { return pick(tok, m, mk<V>(1, 0)); }
+template <class V> __device__ __forceinline__ V zero_unless(bool tok, V m) { return pick(tok, m, mk<V>(0, 0)); }
+
+template <bool MS, class V, int B>
+__device__ __forceinline__ void op_u1(Regs<V> &v, V m0, V m1, V m2, V m3, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & (1 << B)) continue;
-        if ((uint32_t(j) & creg) != creg || !tok) continue;
-        const int j1 = j | (1 << B);
-        const V a0 = v[j], a1 = v[j1];
-        v[j] = cmac2(m0, a0, m1, a1);
-        v[j1] = cmac2(m2, a0, m3, a1);
+        if ((uint32_t(j) & creg) != creg) continue;
+        mix2<MS>(v[j], v[j | (1 << B)], m0, m1, m2, m3);
     }
 }
 
@@ -76,7 +110,20 @@ template <class V, int B> __device__ __forceinline__ void op_cx(Regs<V> &v, uint
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & (1 << B)) continue;
-        if ((uint32_t(j) & creg) != creg || !tok) continue;
+        if ((uint32_t(j) & creg) != creg) continue;
+        const int j1 = j | (1 << B);
+        const V a0 = v[j], a1 = v[j1];
+        v[j] = pick(tok, a1, a0);
+        v[j1] = pick(tok, a0, a1);
+        }
+}
+
+template <class V, int B> __device__ __forceinline__ void op_x(Regs<V> &v, uint32_t creg)
+{
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        if (j & (1 << B)) continue;
+        if ((uint32_t(j) & creg) != creg) continue;
         const int j1 = j | (1 << B);
         const V t = v[j];
         v[j] = v[j1];
@@ -84,40 +131,33 @@ template <class V, int B> __device__ __forceinline__ void op_cx(Regs<V> &v, uint
     }
 }
 
-template <class V, int B>
-__device__ __forceinline__ void op_diagr(Regs<V> &v, V d0, V d1, bool d0_one, uint32_t creg, bool tok)
+template <bool MS, class V, int B>
+__device__ __forceinline__ void op_diagr(Regs<V> &v, V d0, V d1, bool d0_one, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
-        if ((uint32_t(j) & creg) != creg || !tok) continue;
-        if (j & (1 << B)) {
-            v[j] = cmul(d1, v[j]);
-        } else if (!d0_one) {
-            v[j] = cmul(d0, v[j]);
-        }
+        if ((uint32_t(j) & creg) != creg) continue;
+        if (j & (1 << B)) scale_by<MS>(v[j], d1);
+        else if (!d0_one) scale_by<MS>(v[j], d0);
     }
 }
 
-template <class V> __device__ __forceinline__ void op_diagt(Regs<V> &v, V d, uint32_t creg, bool tok)
+template <class V> __device__ __forceinline__ void op_diagt(Regs<V> &v, V d, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
-        if ((uint32_t(j) & creg) != creg || !tok) continue;
-        v[j] = cmul(d, v[j]);
+        if ((uint32_t(j) & creg) != creg) continue;
+        scale_by<false>(v[j], d);
     }
 }
 
 // Dense 4x4 in the basis |b_hi b_lo> (b_hi = MSB).
-template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(Regs<V> &v, const V *__restrict__ m)
+template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(Regs<V> &v, const V *m)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & ((1 << BH) | (1 << BL))) continue;
-        const int i0 = j, i1 = j | (1 << BL), i2 = j | (1 << BH), i3 = j | (1 << BH) | (1 << BL);
-        const V a0 = v[i0], a1 = v[i1], a2 = v[i2], a3 = v[i3];
-        const int idx[4] = {i0, i1, i2, i3};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) v[idx[r]] = cmac4(m + 4 * r, a0, a1, a2, a3);
+        mix4(v[j], v[j | (1 << BL)], v[j | (1 << BH)], v[j | (1 << BH) | (1 << BL)], m);
     }
 }
 
@@ -141,6 +181,14 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
     default: fn<V, 3>(__VA_ARGS__); break;  \
     }
 
+#define EV_CASE_B_MS(fn, MS, ...)                \
+    switch (b0) {                                \
+    case 0: fn<MS, V, 0>(__VA_ARGS__); break;    \
+    case 1: fn<MS, V, 1>(__VA_ARGS__); break;    \
+    case 2: fn<MS, V, 2>(__VA_ARGS__); break;    \
+    default: fn<MS, V, 3>(__VA_ARGS__); break;   \
+    }
+
 #define EV_CASE_PAIR(fn, ...)                        \
     switch (b0 * 4 + b1) {                           \
     case 1 * 4 + 0: fn<V, 1, 0>(__VA_ARGS__); break; \
@@ -151,138 +199,286 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
     default: fn<V, 3, 2>(__VA_ARGS__); break;        \
     }
 
+// Descriptors are device memory that the host fills before a launch and no
+// kernel writes, so kernels read them through the constant address space:
+// uniform reads become scalar loads, which the compiler may keep in flight
+// across the amplitude stores.
+#define EV_CONST __attribute__((address_space(4)))
+template <class T> using cptr = const EV_CONST T *;
+template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
+
+// The fields of one op descriptor that the op bodies read, held in scalar
+// registers.  The op loop fetches the next op's view while the current op
+// runs, so the scalar-cache latency hides behind the arithmetic.
+template <class V> struct OpView {
+    uint32_t code, creg;
+    uint64_t cglob;
+    uint32_t coef, tq;
+    V m[4];
+};
+
+template <class V> __device__ __forceinline__ OpView<V> fetch_op(cptr<DevOp> p)
+{
+    OpView<V> o;
+    o.code = p->code;
+    o.creg = p->creg;
+    o.cglob = p->cglob;
+    o.coef = p->coef;
+    o.tq = p->tq;
+    using F = decltype(V::x);
+    const cptr<F> f = (cptr<F>)(p->m);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o.m[k] = mk<V>(f[2 * k], f[2 * k + 1]);
+    return o;
+}
+
+template <class V> __device__ __forceinline__ void load_row(cptr<V> m, V *out)
+{
+    using F = decltype(V::x);
+    const cptr<F> f = (cptr<F>)(m);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[k] = mk<V>(f[2 * k], f[2 * k + 1]);
+}
+
 template <class V>
-__device__ __forceinline__ void run_op(const DevOp &op, Regs<V> &v, uint64_t gfix, const V *__restrict__ coef)
+__device__ __forceinline__ void run_op(const OpView<V> &op, Regs<V> &v, uint64_t gfix, cptr<V> coef)
 {
     const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
     const uint32_t flags = op.code >> 24;
-    const V *m = (kind == kU2) ? coef + op.coef : reinterpret_cast<const V *>(op.m);
     const bool tok = (gfix & op.cglob) == op.cglob;
     switch (kind) {
-    case kU1: {
-        const V m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
-        EV_CASE_B(op_u1, v, m0, m1, m2, m3);
+    case kU1: EV_CASE_B_MS(op_u1, true, v, op.m[0], op.m[1], op.m[2], op.m[3], 0u); break;
+    case kCU1:
+        if (op.cglob == 0) {
+            EV_CASE_B_MS(op_u1, true, v, op.m[0], op.m[1], op.m[2], op.m[3], op.creg);
+        } else {
+            const V m0 = one_unless(tok, op.m[0]), m1 = zero_unless(tok, op.m[1]);
+            const V m2 = zero_unless(tok, op.m[2]), m3 = one_unless(tok, op.m[3]);
+            EV_CASE_B_MS(op_u1, false, v, m0, m1, m2, m3, op.creg);
+        }
         break;
-    }
-    case kCU1: {
-        const V m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
-        EV_CASE_B(op_cu1, v, m0, m1, m2, m3, op.creg, tok);
+    case kCX:
+        if (op.cglob == 0) {
+            EV_CASE_B(op_x, v, op.creg);
+        } else {
+            EV_CASE_B(op_cx, v, op.creg, tok);
+        }
         break;
-    }
-    case kCX: EV_CASE_B(op_cx, v, op.creg, tok); break;
     case kDiagR: {
-        const V d0 = m[0], d1 = m[1];
         const bool one = (flags & kD0IsOne) != 0;
-        EV_CASE_B(op_diagr, v, d0, d1, one, op.creg, tok);
+        if (op.cglob == 0) {
+            EV_CASE_B_MS(op_diagr, true, v, op.m[0], op.m[1], one, op.creg);
+        } else {
+            const V d0 = one_unless(tok, op.m[0]), d1 = one_unless(tok, op.m[1]);
+            EV_CASE_B_MS(op_diagr, false, v, d0, d1, one, op.creg);
+        }
         break;
     }
     case kDiagT: {
-        const V d = ((gfix >> op.tq) & 1) ? m[1] : m[0];
-        op_diagt<V>(v, d, op.creg, tok);
+        const V d = one_unless(tok, pick(((gfix >> op.tq) & 1) != 0, op.m[1], op.m[0]));
+        op_diagt<V>(v, d, op.creg);
         break;
     }
-    case kU2: EV_CASE_PAIR(op_u2, v, m); break;
+    case kU2: {
+        V m[16];
+        load_row<V>(coef + op.coef, m);
+        EV_CASE_PAIR(op_u2, v, m);
+        break;
+    }
     default: EV_CASE_PAIR(op_swap, v); break;
     }
 }
 
-// One tile of one pass.  Threads beyond the tile's 2^(t-4) (a batch launch
-// sizes its workgroups for the largest tile) skip the data movement but keep
-// every barrier.
+// One tile's place in its pass: the state, the pass program, and the global
+// index bits of the tile (the tile index deposited into the non-tile qubits).
+template <class R> struct TileRef {
+    Amp<R> *psi;
+    cptr<Stage> stages;
+    cptr<DevOp> ops;
+    cptr<Amp<R>> coef;
+    uint64_t base;
+    int t, nstages;
+};
+
+// Inserts a zero bit at every tile qubit (ascending), which spreads the tile
+// index over the non-tile qubits.
+__device__ __forceinline__ uint64_t tile_base(cptr<Pass> pass, uint64_t T)
+{
+    uint64_t b = T;
+    for (int i = 0; i < pass->t; ++i) b = insert0(b, pass->q[i]);
+    return b;
+}
+
 template <class R>
-__device__ __forceinline__ void run_tile(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
-                                         const Stage *__restrict__ stages, const DevOp *__restrict__ ops,
-                                         const Amp<R> *__restrict__ coef, uint64_t T, Amp<R> *lds)
+__device__ __forceinline__ TileRef<R> tile_ref(Amp<R> *psi, cptr<Pass> pass, cptr<Stage> stages, cptr<DevOp> ops,
+                                               cptr<Amp<R>> coef, uint64_t T)
+{
+    TileRef<R> r;
+    r.psi = psi;
+    r.stages = stages + pass->stage_begin;
+    r.ops = ops;
+    r.coef = coef;
+    r.base = tile_base(pass, T);
+    r.t = pass->t;
+    r.nstages = pass->nstages;
+    return r;
+}
+
+// A thread's share of a stage mapping: its thread-id bits placed at the
+// stage's non-register positions, as a swizzled LDS slot and as global
+// index bits (tile base included).
+struct Lane {
+    uint32_t sb;
+    uint64_t gfix;
+};
+
+__device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base)
+{
+    uint32_t lb = 0;
+    uint64_t g = base;
+    for (int i = 0; i < st->np; ++i)
+        if ((tid >> i) & 1u) {
+            lb |= 1u << st->p[i];
+            g |= uint64_t(1) << st->gpos[i];
+        }
+    return {swz(lb), g};
+}
+
+// Global gather of a tile in its first stage's mapping.  The planner keeps
+// that stage's register bits off the lowest local positions, so 2^3 (c128)
+// or 2^4 (c64) consecutive lanes read one contiguous 128-byte run.
+template <class R>
+__device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v)
+{
+    const cptr<Stage> st = tr.stages;
+    const Lane ln = lane_of(st, tid, tr.base);
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        v[j] = tr.psi[ln.gfix | st->goff[j]];
+        }
+}
+
+template <class V>
+__device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, uint64_t gfix, cptr<DevOp> ops, cptr<V> coef)
+{
+    const int ob = __builtin_amdgcn_readfirstlane(st->op_begin), oe = __builtin_amdgcn_readfirstlane(st->op_end);
+    if (ob >= oe) return;
+    OpView<V> cur = fetch_op<V>(ops + ob);
+    for (int o = ob; o < oe; ++o) {
+        const OpView<V> nxt = fetch_op<V>(ops + ((o + 1 < oe) ? o + 1 : o));
+        run_op<V>(cur, v, gfix, coef);
+        cur = nxt;
+    }
+}
+
+// Runs one tile whose first-stage amplitudes are already in v: stages with
+// LDS exchanges between them, then the global scatter in the last stage's
+// mapping.  Threads beyond the tile (a batch launch sizes workgroups for its
+// widest tile) run the register arithmetic on don't-care values but skip
+// every memory access, and keep every barrier; the op loop thus stays out of
+// divergent control flow and its descriptor reads stay scalar.
+template <class R>
+__device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v, Amp<R> *lds)
 {
     using V = Amp<R>;
-    const int t = pass->t;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t W = 1u << (t - kRegBits);
-    const bool active = tid < W;
-
-    // Tile base: deposit the tile index into the non-tile qubits.
-    uint64_t base = 0;
-    for (int i = 0; i < pass->ncomp; ++i) base |= ((T >> i) & 1ull) << pass->comp[i];
-
-    // Load: local index l = j*W + tid (lanes on consecutive amplitudes).
-    uint64_t gthr = 0;
-    for (int i = 0; i < t - kRegBits; ++i)
-        if ((tid >> i) & 1u) gthr |= 1ull << pass->q[i];
-    if (active) {
-#pragma unroll
-        for (int j = 0; j < kRegs; ++j) {
-            uint64_t gj = 0;
-#pragma unroll
-            for (int k = 0; k < kRegBits; ++k)
-                if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
-            lds[swz(uint32_t(j) * W + tid)] = psi[base | gthr | gj];
-        }
-    }
-    __syncthreads();
-
-    const int nst = pass->nstages;
+    const bool active = tid < (1u << (tr.t - kRegBits));
+    const int nst = tr.nstages;
     for (int s = 0; s < nst; ++s) {
-        const Stage &st = stages[pass->stage_begin + s];
-        if (active) {
-            uint32_t lb = 0;
-            uint64_t gfix = base;
-            for (int i = 0; i < st.np; ++i)
-                if ((tid >> i) & 1u) {
-                    lb |= 1u << st.p[i];
-                    gfix |= 1ull << st.gpos[i];
-                }
-            const uint32_t sb = swz(lb);
-            Regs<V> v;
+        const cptr<Stage> st = tr.stages + s;
+        const Lane ln = lane_of(st, tid, tr.base);
+        if (s > 0 && active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) v[j] = lds[sb ^ st.roff[j]];
-            for (int o = st.op_begin; o < st.op_end; ++o) run_op<V>(ops[o], v, gfix, coef);
-#pragma unroll
-            for (int j = 0; j < kRegs; ++j) lds[sb ^ st.roff[j]] = v[j];
+            for (int j = 0; j < kRegs; ++j) v[j] = lds[ln.sb ^ st->roff[j]];
         }
-        __syncthreads();
-    }
-
-    if (active) {
+        run_stage_ops<V>(st, v, ln.gfix, tr.ops, tr.coef);
+        if (s + 1 < nst) {
+            if (active) {
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) {
-            uint64_t gj = 0;
+                for (int j = 0; j < kRegs; ++j) lds[ln.sb ^ st->roff[j]] = v[j];
+            }
+            lds_barrier();
+        } else if (active) {
 #pragma unroll
-            for (int k = 0; k < kRegBits; ++k)
-                if ((j >> k) & 1) gj |= 1ull << pass->q[t - kRegBits + k];
-            psi[base | gthr | gj] = lds[swz(uint32_t(j) * W + tid)];
+            for (int j = 0; j < kRegs; ++j) {
+                tr.psi[ln.gfix | st->goff[j]] = v[j];
+                        }
         }
     }
 }
 
+// One workgroup per tile: gather the tile in its first stage's mapping, run
+// the stages, scatter it.  Workgroups are short-lived and the hardware
+// dispatcher keeps every CU full; a persistent grid that loops over tiles
+// measured 1.7x slower on MI355X (any CU slot it fails to place delays a
+// whole strip of tiles).
 template <class R>
-__global__ __launch_bounds__(kMaxThreads) void k_fused_pass(Amp<R> *__restrict__ psi, const Pass *__restrict__ pass,
-                                                            const Stage *__restrict__ stages,
-                                                            const DevOp *__restrict__ ops,
-                                                            const Amp<R> *__restrict__ coef)
+__device__ __forceinline__ void one_tile(const TileRef<R> &tr, Amp<R> *lds)
 {
-    extern __shared__ unsigned char lds_raw[];
-    run_tile<R>(psi, pass, stages, ops, coef, blockIdx.x, reinterpret_cast<Amp<R> *>(lds_raw));
+    using V = Amp<R>;
+    const uint32_t tid = threadIdx.x;
+    Regs<V> v;
+    load_tile<R>(tr, tid, v);  // threads beyond the tile read an in-tile alias (their bits above t-4 are ignored)
+    compute_tile<R>(tr, tid, v, lds);
 }
 
-// Persistent batch kernel: linear tile ids over every entry of the round
-// (entries sorted by tile_begin); workgroups stride over them.
+// Register budget: 168 VGPRs (3 waves per SIMD).  A c128 tile of 2^10
+// amplitudes takes 16 KiB of LDS, so LDS allows 10 single-wave workgroups
+// per CU (2.5 per SIMD); registers are not the tighter limit.
+template <class R> constexpr int kWavesPerSimd = 3;
+
+// XCD-aware tile order: dispatch places workgroup b on XCD b mod 8, so XCD x
+// gets the contiguous tile range [x * tiles/8, (x+1) * tiles/8), and tiles
+// that neighbour in memory share an L2.
 template <class R>
-__global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *__restrict__ entries, int nentries,
-                                                             uint64_t total_tiles)
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_pass(
+    Amp<R> *__restrict__ psi, const Pass *__restrict__ pass, const Stage *__restrict__ stages,
+    const DevOp *__restrict__ ops, const Amp<R> *__restrict__ coef, uint64_t tiles, uint32_t xcds)
 {
     extern __shared__ unsigned char lds_raw[];
-    Amp<R> *lds = reinterpret_cast<Amp<R> *>(lds_raw);
-    for (uint64_t lin = blockIdx.x; lin < total_tiles; lin += gridDim.x) {
-        int lo = 0, hi = nentries - 1;  // last entry with tile_begin <= lin
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) / 2;
-            if (entries[mid].tile_begin <= lin) lo = mid; else hi = mid - 1;
-        }
-        const BatchEntry &e = entries[lo];
-        run_tile<R>(static_cast<Amp<R> *>(e.psi), e.pass, e.stages, e.ops, static_cast<const Amp<R> *>(e.coef),
-                    lin - e.tile_begin, lds);
-        __syncthreads();  // the next tile's load overwrites this tile's LDS
+    const uint64_t per = tiles / xcds;
+    const uint64_t T = (blockIdx.x % xcds) * per + blockIdx.x / xcds;
+    one_tile<R>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T),
+                reinterpret_cast<Amp<R> *>(lds_raw));
+}
+
+// Batch kernel: workgroup b runs linear tile b of the round, whose entry
+// (entries sorted by tile_begin) is found by binary search.
+template <class R>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_batch(
+    const BatchEntry *__restrict__ entries, int nentries)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const cptr<BatchEntry> ent = as_const(entries);
+    const uint64_t lin = blockIdx.x;
+    int lo = 0, hi = nentries - 1;  // last entry with tile_begin <= lin
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (ent[mid].tile_begin <= lin) lo = mid; else hi = mid - 1;
     }
+    const cptr<BatchEntry> e = ent + lo;
+    one_tile<R>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
+                            as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin),
+                reinterpret_cast<Amp<R> *>(lds_raw));
+}
+
+template <class R>
+void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops, const void *coef,
+                   hipStream_t st)
+{
+    const uint64_t tiles = uint64_t(1) << (n - t);
+    const unsigned block = 1u << (t - kRegBits);  // <= kMaxThreads = the kernel's launch bound
+    const size_t lds = (size_t(1) << t) * sizeof(Amp<R>);
+    const uint32_t xcds = (tiles % 8 == 0) ? 8u : 1u;
+    hipLaunchKernelGGL(k_fused_pass<R>, dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
+                       stages, ops, static_cast<const Amp<R> *>(coef), tiles, xcds);
+}
+
+template <class R>
+void launch_batch_t(const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st)
+{
+    const unsigned block = 1u << (max_t - kRegBits);
+    const size_t lds = (size_t(1) << max_t) * sizeof(Amp<R>);
+    hipLaunchKernelGGL(k_fused_batch<R>, dim3(unsigned(total_tiles)), dim3(block), lds, st, entries, nentries);
 }
 
 }  // namespace
@@ -291,32 +487,18 @@ void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stag
                  const void *coef, hipStream_t st)
 {
     EV_REQUIRE(t >= kMinTile && t <= kMaxTile && t <= n, EV_E_ARG, "fused pass tile width out of range");
-    const dim3 grid(static_cast<unsigned>(uint64_t(1) << (n - t)));
-    const dim3 block(1u << (t - kRegBits));  // <= kMaxThreads = the kernel's launch bound
-    if (!c64) {
-        const size_t lds = (size_t(1) << t) * sizeof(double2);
-        hipLaunchKernelGGL(k_fused_pass<double>, grid, block, lds, st, static_cast<double2 *>(psi), pass, stages, ops,
-                           static_cast<const double2 *>(coef));
-    } else {
-        const size_t lds = (size_t(1) << t) * sizeof(float2);
-        hipLaunchKernelGGL(k_fused_pass<float>, grid, block, lds, st, static_cast<float2 *>(psi), pass, stages, ops,
-                           static_cast<const float2 *>(coef));
-    }
+    EV_REQUIRE(n - t < 32, EV_E_LIMIT, "fused pass has more than 2^32 - 1 tiles");
+    if (!c64) launch_pass_t<double>(psi, n, t, pass, stages, ops, coef, st);
+    else launch_pass_t<float>(psi, n, t, pass, stages, ops, coef, st);
 }
 
-void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, unsigned grid,
-                  hipStream_t st)
+void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st)
 {
     EV_REQUIRE(max_t >= kMinTile && max_t <= kMaxTile, EV_E_ARG, "batch tile width out of range");
+    EV_REQUIRE(total_tiles < (uint64_t(1) << 32), EV_E_LIMIT, "batch round has more than 2^32 - 1 tiles");
     if (nentries == 0 || total_tiles == 0) return;
-    const dim3 block(1u << (max_t - kRegBits));
-    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(grid, total_tiles)));
-    if (!c64)
-        hipLaunchKernelGGL(k_fused_batch<double>, g, block, (size_t(1) << max_t) * sizeof(double2), st, entries, nentries,
-                           total_tiles);
-    else
-        hipLaunchKernelGGL(k_fused_batch<float>, g, block, (size_t(1) << max_t) * sizeof(float2), st, entries, nentries,
-                           total_tiles);
+    if (!c64) launch_batch_t<double>(entries, nentries, total_tiles, max_t, st);
+    else launch_batch_t<float>(entries, nentries, total_tiles, max_t, st);
 }
 
 }  // namespace ev::fused

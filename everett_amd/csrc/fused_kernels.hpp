@@ -8,8 +8,9 @@
 namespace ev::fused {
 
 // One launch = one pass over a 2^n state with tile width t: 2^(n-t)
-// workgroups of 2^(t-4) threads.  Descriptor pointers are device memory;
-// coef holds double2 (c128) or float2 (c64) entries.
+// workgroups of 2^(t-4) threads, one tile of 2^t amplitudes each.
+// Descriptor pointers are device memory; coef holds double2 (c128) or float2
+// (c64) entries.
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
                  const void *coef, hipStream_t st);
 
@@ -24,9 +25,8 @@ struct BatchEntry {
     uint64_t tile_begin;
 };
 
-// One persistent launch over every entry of a round: `grid` workgroups of
-// 2^(max_t-4) threads stride over the total_tiles linear tile ids.
-void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, unsigned grid,
-                  hipStream_t st);
+// One launch over every entry of a round: total_tiles workgroups of
+// 2^(max_t-4) threads, workgroup b running linear tile b.
+void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st);
 
 }  // namespace ev::fused

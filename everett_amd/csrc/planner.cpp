@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "dm_ops.hpp"
@@ -177,9 +178,402 @@ int popc(uint64_t x) { return __builtin_popcountll(x); }
 int rank_in(uint64_t mask, int bit) { return popc(mask & ((uint64_t(1) << bit) - 1)); }
 
 // Builds the pass list for `g` on an n-qubit state.
-Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t, int low, float budget)
+// ---- pair fusion inside a stage ------------------------------------------
+//
+// Every op in a stage costs a dispatch and register shuffling in the kernel
+// beyond its arithmetic, so ops confined to two register qubits are fused
+// into one dense 4x4 (G_U2) when that does not add arithmetic: a 4x4 costs 16
+// FMA per amplitude, the same as two 2x2s, so U1(a) U1(b) CX(a,b) -- the
+// brick-wall pattern -- becomes one op.  Greedy in stage order, with one
+// "open" block per register qubit (the latest block touching it): an op on
+// qubits whose open blocks can take it is multiplied into them; ops that do
+// not touch a qubit commute with it, so absorbing moves nothing across a
+// dependency.  Products are formed in f64 on the host.
+
+// Cost of an op in the kernel, in FMA per amplitude plus a per-op overhead.
+constexpr float kOpOverhead = 4.0f;
+float fused_cost(GKind k)
+{
+    switch (k) {
+    case G_U1:
+    case G_CU1: return 8.0f + kOpOverhead;
+    case G_U2: return 16.0f + kOpOverhead;
+    case G_DIAG: return 4.0f + kOpOverhead;
+    default: return 2.0f + kOpOverhead;  // register moves
+    }
+}
+
+Mat4 mul4(const Mat4 &a, const Mat4 &b)  // a * b
+{
+    Mat4 r{};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            c128 acc{0, 0};
+            for (int k = 0; k < 4; ++k) acc = cadd(acc, cmul(a[i * 4 + k], b[k * 4 + j]));
+            r[i * 4 + j] = acc;
+        }
+    return r;
+}
+
+// A 2x2 on the MSB (hi = true) or LSB of a pair, as a 4x4.
+Mat4 lift(const Mat2 &m, bool hi)
+{
+    Mat4 r{};
+    for (int ra = 0; ra < 2; ++ra)
+        for (int rb = 0; rb < 2; ++rb)
+            for (int ca = 0; ca < 2; ++ca)
+                for (int cb = 0; cb < 2; ++cb) {
+                    c128 v{0, 0};
+                    if (hi && rb == cb) v = m[ra * 2 + ca];
+                    if (!hi && ra == ca) v = m[rb * 2 + cb];
+                    r[(ra * 2 + rb) * 4 + ca * 2 + cb] = v;
+                }
+    return r;
+}
+
+struct Block {
+    enum Kind { Single, Pair, Opaque, Dead } kind;
+    int a = -1, b = -1;  // Single: a; Pair: a = MSB, b = LSB of the basis
+    Mat2 m2{};
+    Mat4 m4{};
+    int op = -1;  // Opaque: index into g
+    float cost = 0;
+};
+
+class StageFuser {
+  public:
+    StageFuser(const std::vector<GOp> &g, const std::vector<c128> &pool, uint64_t regq) : g_(g), pool_(pool), regq_(regq)
+    {
+    }
+
+    void add(int id)
+    {
+        const GOp &op = g_[id];
+        Mat2 m2;
+        Mat4 m4;
+        int a, b;
+        if (as_single(op, &a, &m2)) return add_single(id, a, m2);
+        if (as_pair(op, &a, &b, &m4)) return add_pair(id, a, b, m4);
+        add_opaque(id);
+    }
+
+    // The fused stage as planner ops; new dense 4x4 and 2x2 coefficients go to pool.
+    std::vector<GOp> finish(std::vector<c128> &pool) const
+    {
+        std::vector<GOp> out;
+        for (const Block &bl : blocks_) {
+            switch (bl.kind) {
+            case Block::Dead: break;
+            case Block::Opaque: out.push_back(g_[bl.op]); break;
+            case Block::Single: {
+                if (is_identity2(bl.m2)) break;
+                GOp o{};
+                o.q0 = bl.a;
+                o.coef = uint32_t(pool.size());
+                if (classify1(bl.m2.data()) != K1::Dense) {
+                    o.kind = G_DIAG;
+                    pool.push_back(bl.m2[0]);
+                    pool.push_back(bl.m2[3]);
+                } else {
+                    o.kind = G_U1;
+                    pool.insert(pool.end(), bl.m2.begin(), bl.m2.end());
+                }
+                out.push_back(o);
+                break;
+            }
+            case Block::Pair: {
+                if (bl.op >= 0) {  // a lone op that nothing joined: keep its cheaper form
+                    out.push_back(g_[bl.op]);
+                    break;
+                }
+                GOp o{};
+                o.kind = G_U2;
+                o.q0 = bl.a;
+                o.q1 = bl.b;
+                o.coef = uint32_t(pool.size());
+                pool.insert(pool.end(), bl.m4.begin(), bl.m4.end());
+                out.push_back(o);
+                break;
+            }
+            }
+        }
+        return out;
+    }
+
+  private:
+    static bool is_identity2(const Mat2 &m)
+    {
+        return ceq(m[0], c128{1, 0}) && ceq(m[1], c128{0, 0}) && ceq(m[2], c128{0, 0}) && ceq(m[3], c128{1, 0});
+    }
+    bool reg(int q) const { return q >= 0 && ((regq_ >> q) & 1); }
+
+    // One-qubit ops on a register qubit: dense or diagonal 2x2.
+    bool as_single(const GOp &op, int *q, Mat2 *m) const
+    {
+        if (op.cmask || !reg(op.q0)) return false;
+        const c128 *c = pool_.data() + op.coef;
+        if (op.kind == G_U1) *m = Mat2{c[0], c[1], c[2], c[3]};
+        else if (op.kind == G_DIAG) *m = Mat2{c[0], c128{0, 0}, c128{0, 0}, c[1]};
+        else return false;
+        *q = op.q0;
+        return true;
+    }
+
+    // Two-qubit ops confined to register qubits, as a 4x4 in the basis |a b>.
+    bool as_pair(const GOp &op, int *a, int *b, Mat4 *m) const
+    {
+        const c128 *c = pool_.data() + op.coef;
+        const c128 one{1, 0}, zero{0, 0};
+        *m = Mat4{};
+        if (op.kind == G_U2 || op.kind == G_SWAP) {
+            if (!reg(op.q0) || !reg(op.q1) || op.cmask) return false;
+            *a = op.q0;
+            *b = op.q1;
+            if (op.kind == G_U2) std::copy(c, c + 16, m->begin());
+            else *m = gate::swap();
+            return true;
+        }
+        if (__builtin_popcountll(op.cmask) != 1 || !reg(op.q0)) return false;
+        const int ctl = __builtin_ctzll(op.cmask);
+        if (!reg(ctl)) return false;
+        *a = ctl;  // control = MSB of the basis
+        *b = op.q0;
+        Mat2 t;
+        switch (op.kind) {
+        case G_CX: t = Mat2{zero, one, one, zero}; break;
+        case G_CU1: t = Mat2{c[0], c[1], c[2], c[3]}; break;
+        case G_DIAG: t = Mat2{c[0], zero, zero, c[1]}; break;
+        default: return false;
+        }
+        for (int i = 0; i < 2; ++i) (*m)[i * 5] = one;  // control 0: identity on the target
+        for (int r = 0; r < 2; ++r)
+            for (int cc = 0; cc < 2; ++cc) (*m)[(2 + r) * 4 + 2 + cc] = t[r * 2 + cc];
+        return true;
+    }
+
+    int open_of(int q) const
+    {
+        auto it = open_.find(q);
+        return it == open_.end() ? -1 : it->second;
+    }
+
+    void add_single(int id, int q, const Mat2 &m)
+    {
+        const int o = open_of(q);
+        if (o >= 0 && blocks_[o].kind == Block::Single) {
+            blocks_[o].m2 = gate::compose(m, blocks_[o].m2);
+            return;
+        }
+        if (o >= 0 && blocks_[o].kind == Block::Pair) {
+            Block &bl = blocks_[o];
+            bl.m4 = mul4(lift(m, q == bl.a), bl.m4);
+            bl.op = -1;
+            return;
+        }
+        Block bl{Block::Single};
+        bl.a = q;
+        bl.m2 = m;
+        bl.cost = fused_cost(g_[id].kind);
+        open_[q] = int(blocks_.size());
+        blocks_.push_back(bl);
+    }
+
+    void add_pair(int id, int a, int b, const Mat4 &m)
+    {
+        const int oa = open_of(a), ob = open_of(b);
+        if (oa >= 0 && oa == ob && blocks_[oa].kind == Block::Pair) {  // same pair: multiply in
+            Block &bl = blocks_[oa];
+            Mat4 mm = m;
+            if (bl.a != a) mm = reorder(m);
+            bl.m4 = mul4(mm, bl.m4);
+            bl.op = -1;
+            return;
+        }
+        const bool sa = oa >= 0 && blocks_[oa].kind == Block::Single;
+        const bool sb = ob >= 0 && blocks_[ob].kind == Block::Single;
+        const float parts = fused_cost(g_[id].kind) + (sa ? blocks_[oa].cost : 0) + (sb ? blocks_[ob].cost : 0);
+        const bool dense = g_[id].kind == G_U2;
+        Block bl{Block::Pair};
+        bl.a = a;
+        bl.b = b;
+        bl.cost = fused_cost(G_U2);
+        if (dense || parts >= bl.cost) {
+            Mat4 pre = lift(gate::id(), true);
+            if (sa) {
+                pre = mul4(lift(blocks_[oa].m2, true), pre);
+                blocks_[oa].kind = Block::Dead;
+            }
+            if (sb) {
+                pre = mul4(lift(blocks_[ob].m2, false), pre);
+                blocks_[ob].kind = Block::Dead;
+            }
+            bl.m4 = mul4(m, pre);
+            bl.op = (sa || sb) ? -1 : id;
+        } else {  // not worth a dense 4x4: keep the op, open for both qubits
+            bl.kind = Block::Opaque;
+            bl.op = id;
+        }
+        open_[a] = open_[b] = int(blocks_.size());
+        blocks_.push_back(bl);
+    }
+
+    void add_opaque(int id)
+    {
+        Block bl{Block::Opaque};
+        bl.op = id;
+        const int idx = int(blocks_.size());
+        for (uint64_t t = g_[id].touched(); t; t &= t - 1) open_[__builtin_ctzll(t)] = idx;
+        blocks_.push_back(bl);
+    }
+
+    // 4x4 in the basis |b a> -> |a b>.
+    static Mat4 reorder(const Mat4 &m)
+    {
+        auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
+        Mat4 r{};
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) r[sw(i) * 4 + sw(j)] = m[i * 4 + j];
+        return r;
+    }
+
+    const std::vector<GOp> &g_;
+    const std::vector<c128> &pool_;
+    uint64_t regq_;
+    std::vector<Block> blocks_;
+    std::map<int, int> open_;
+};
+
+// Lowers one planner op into a device op for register set S (local
+// positions); regbit maps a global qubit to its register bit or -1.
+template <class RegBit>
+DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const RegBit &regbit, std::vector<c128> &coef)
+{
+    DevOp d{};
+    uint32_t kind = 0, b0 = 0, b1 = 0, flags = 0;
+    for (uint64_t m = op.cmask; m; m &= m - 1) {
+        const int q = __builtin_ctzll(m);
+        const int rb = regbit(S, q);
+        if (rb >= 0) d.creg |= 1u << rb;
+        else d.cglob |= uint64_t(1) << q;
+    }
+    d.coef = uint32_t(coef.size());
+    switch (op.kind) {
+    case G_U1:
+    case G_CU1:
+        kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
+        b0 = uint32_t(regbit(S, op.q0));
+        for (int e = 0; e < 4; ++e) {
+            d.m[2 * e] = pool[op.coef + e].re;
+            d.m[2 * e + 1] = pool[op.coef + e].im;
+        }
+        break;
+    case G_CX:
+        kind = fused::kCX;
+        b0 = uint32_t(regbit(S, op.q0));
+        break;
+    case G_DIAG: {
+        const c128 d0 = pool[op.coef], d1 = pool[op.coef + 1];
+        const int rb = regbit(S, op.q0);
+        if (rb >= 0) {
+            kind = fused::kDiagR;
+            b0 = uint32_t(rb);
+        } else {
+            kind = fused::kDiagT;
+            d.tq = uint32_t(op.q0);
+        }
+        if (d0.re == 1 && d0.im == 0) flags |= fused::kD0IsOne;
+        d.m[0] = d0.re;
+        d.m[1] = d0.im;
+        d.m[2] = d1.re;
+        d.m[3] = d1.im;
+        break;
+    }
+    case G_U2: {
+        kind = fused::kU2;
+        const int ra = regbit(S, op.q0), rb = regbit(S, op.q1);
+        const c128 *m = pool.data() + op.coef;
+        if (ra > rb) {
+            b0 = uint32_t(ra);
+            b1 = uint32_t(rb);
+            coef.insert(coef.end(), m, m + 16);
+        } else {  // reorder the basis |q_a q_b> -> |q_b q_a>
+            b0 = uint32_t(rb);
+            b1 = uint32_t(ra);
+            auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
+            c128 mm[16];
+            for (int r = 0; r < 4; ++r)
+                for (int cc = 0; cc < 4; ++cc) mm[sw(r) * 4 + sw(cc)] = m[r * 4 + cc];
+            coef.insert(coef.end(), mm, mm + 16);
+        }
+        break;
+    }
+    case G_SWAP: {
+        kind = fused::kSwap;
+        const int ra = regbit(S, op.q0), rb = regbit(S, op.q1);
+        b0 = uint32_t(std::max(ra, rb));
+        b1 = uint32_t(std::min(ra, rb));
+        break;
+    }
+    }
+    d.code = fused::op_code(kind, b0, b1, flags);
+    return d;
+}
+
+// Register positions for a stage that needs `S`: padded to 4 with free
+// positions, never from `avoid` when the stage must stay coalesced, and
+// otherwise scored so that the 16 lanes of an LDS quarter-wave fall on
+// distinct 16-byte slots (thread-bit positions distinct mod 4), then away
+// from the low positions, then as high as possible.
+uint32_t pad_registers(uint32_t S, int t, uint32_t avoid, bool must_avoid)
+{
+    const int need = kRegBits - popc(S);
+    if (need <= 0) return S;
+    int freep[16], nf = 0;
+    for (int l = 0; l < t; ++l)
+        if (!((S >> l) & 1) && !(must_avoid && ((avoid >> l) & 1))) freep[nf++] = l;
+    uint32_t best = 0;
+    long best_score = -1;
+    // every subset of `need` free positions (at most C(12, 4) = 495)
+    int idx[4];
+    for (int k = 0; k < need; ++k) idx[k] = k;
+    while (nf >= need) {
+        uint32_t cand = S;
+        for (int k = 0; k < need; ++k) cand |= 1u << freep[idx[k]];
+        int p[4], np = 0;
+        for (int l = 0; l < t && np < 4; ++l)
+            if (!((cand >> l) & 1)) p[np++] = l;
+        int seen = 0, distinct = 0;
+        for (int k = 0; k < np; ++k)
+            if (!((seen >> (p[k] & 3)) & 1)) {
+                seen |= 1 << (p[k] & 3);
+                ++distinct;
+            }
+        long score = long(distinct) * 1000000 + (popc(cand & avoid) ? 0 : 100000);
+        for (int k = 0; k < need; ++k) score += freep[idx[k]];
+        if (score > best_score) {
+            best_score = score;
+            best = cand;
+        }
+        int k = need - 1;  // next combination
+        while (k >= 0 && idx[k] == nf - need + k) --k;
+        if (k < 0) break;
+        ++idx[k];
+        for (int r = k + 1; r < need; ++r) idx[r] = idx[r - 1] + 1;
+    }
+    EV_REQUIRE(best_score >= 0, EV_E_ARG, "no register positions left for a stage");
+    return best;
+}
+
+// Builds the pass list for `g` on an n-qubit state.  `coalesce` is the number
+// of low qubits whose amplitudes make one 128-byte run (3 for c128, 4 for
+// c64): the first and last stage of every pass keep their register bits off
+// those positions (where the tile holds the state's lowest qubits), so the
+// kernel's global gather and scatter move whole runs.
+Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, int t, int low, float budget,
+          int coalesce)
 {
     Plan P;
+    std::vector<c128> pool = pool_in;  // grows with fused coefficients
     const uint64_t all = (n >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
     const uint64_t lowmask = (uint64_t(1) << low) - 1;
     std::vector<int> rem(g.size());
@@ -229,33 +623,20 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
         ps.stage_begin = int(P.stages.size());
         auto local = [&](int q) { return rank_in(Q, q); };
 
-        // Stages: greedy with 4 register qubits (local positions).
-        std::vector<int> srem = take;
-        while (!srem.empty()) {
-            uint32_t S = 0;
-            uint64_t sblocked = 0;
-            std::vector<int> stake, sdefer;
-            for (int id : srem) {
-                const GOp &op = g[id];
-                const uint64_t tq = op.touched();
-                if (tq & sblocked) {
-                    sblocked |= tq;
-                    sdefer.push_back(id);
-                    continue;
-                }
-                uint32_t need = 0;
-                for (uint64_t m = op.need(); m; m &= m - 1) need |= 1u << local(__builtin_ctzll(m));
-                if (popc(S | need) <= kRegBits) {
-                    S |= need;
-                    stake.push_back(id);
-                } else {
-                    sblocked |= tq;
-                    sdefer.push_back(id);
-                }
-            }
-            srem.swap(sdefer);
-            for (int l = 0; l < t && popc(S) < kRegBits; ++l) S |= 1u << l;
-
+        // register bit of global qubit q under register set S (or -1)
+        auto regbit = [&](uint32_t S, int q) -> int {
+            if (!((Q >> q) & 1)) return -1;
+            const int l = local(q);
+            if (!((S >> l) & 1)) return -1;
+            return rank_in(S, l);
+        };
+        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake) {
+            uint64_t regq = 0;  // global qubits held in registers
+            for (int l = 0; l < t; ++l)
+                if ((S >> l) & 1) regq |= uint64_t(1) << ps.q[l];
+            StageFuser fuser(g, pool, regq);
+            for (int id : stake) fuser.add(id);
+            const std::vector<GOp> fused = fuser.finish(pool);
             Stage st{};
             st.op_begin = int(P.devops.size());
             st.np = t - kRegBits;
@@ -279,88 +660,51 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool, int n, int t
                 st.roff[j] = fused::swz(off);
                 st.goff[j] = goff;
             }
-            // register bit of global qubit q (or -1)
-            auto regbit = [&](int q) -> int {
-                if (!((Q >> q) & 1)) return -1;
-                const int l = local(q);
-                if (!((S >> l) & 1)) return -1;
-                return rank_in(S, l);
-            };
-            for (int id : stake) {
-                const GOp &op = g[id];
-                DevOp d{};
-                uint32_t kind = 0, b0 = 0, b1 = 0, flags = 0;
-                for (uint64_t m = op.cmask; m; m &= m - 1) {
-                    const int q = __builtin_ctzll(m);
-                    const int rb = regbit(q);
-                    if (rb >= 0) d.creg |= 1u << rb;
-                    else d.cglob |= uint64_t(1) << q;
-                }
-                d.coef = uint32_t(P.coef.size());
-                switch (op.kind) {
-                case G_U1:
-                case G_CU1:
-                    kind = (op.kind == G_U1 && op.cmask == 0) ? fused::kU1 : fused::kCU1;
-                    b0 = uint32_t(regbit(op.q0));
-                    for (int e = 0; e < 4; ++e) {
-                        d.m[2 * e] = pool[op.coef + e].re;
-                        d.m[2 * e + 1] = pool[op.coef + e].im;
-                    }
-                    break;
-                case G_CX:
-                    kind = fused::kCX;
-                    b0 = uint32_t(regbit(op.q0));
-                    break;
-                case G_DIAG: {
-                    const c128 d0 = pool[op.coef], d1 = pool[op.coef + 1];
-                    const int rb = regbit(op.q0);
-                    if (rb >= 0) {
-                        kind = fused::kDiagR;
-                        b0 = uint32_t(rb);
-                    } else {
-                        kind = fused::kDiagT;
-                        d.tq = uint32_t(op.q0);
-                    }
-                    if (d0.re == 1 && d0.im == 0) flags |= fused::kD0IsOne;
-                    d.m[0] = d0.re;
-                    d.m[1] = d0.im;
-                    d.m[2] = d1.re;
-                    d.m[3] = d1.im;
-                    break;
-                }
-                case G_U2: {
-                    kind = fused::kU2;
-                    const int ra = regbit(op.q0), rb = regbit(op.q1);
-                    const c128 *m = pool.data() + op.coef;
-                    if (ra > rb) {
-                        b0 = uint32_t(ra);
-                        b1 = uint32_t(rb);
-                        P.coef.insert(P.coef.end(), m, m + 16);
-                    } else {  // reorder the basis |q_a q_b> -> |q_b q_a>
-                        b0 = uint32_t(rb);
-                        b1 = uint32_t(ra);
-                        auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
-                        c128 mm[16];
-                        for (int r = 0; r < 4; ++r)
-                            for (int cc = 0; cc < 4; ++cc) mm[sw(r) * 4 + sw(cc)] = m[r * 4 + cc];
-                        P.coef.insert(P.coef.end(), mm, mm + 16);
-                    }
-                    break;
-                }
-                case G_SWAP: {
-                    kind = fused::kSwap;
-                    const int ra = regbit(op.q0), rb = regbit(op.q1);
-                    b0 = uint32_t(std::max(ra, rb));
-                    b1 = uint32_t(std::min(ra, rb));
-                    break;
-                }
-                }
-                d.code = fused::op_code(kind, b0, b1, flags);
-                P.devops.push_back(d);
-            }
+            for (const GOp &op : fused) P.devops.push_back(lower_op(op, pool, S, regbit, P.coef));
             st.op_end = int(P.devops.size());
             P.stages.push_back(st);
+        };
+
+        // Stages: greedy with 4 register qubits (local positions).  The
+        // first stage takes only ops whose register bits avoid the coalesced
+        // low positions (it may take none: then it only gathers the tile).
+        int contiguous = 0;  // leading tile positions that are the state's lowest qubits
+        while (contiguous < t && ps.q[contiguous] == contiguous) ++contiguous;
+        const uint32_t lowc = (1u << std::min(coalesce, contiguous)) - 1;
+        std::vector<int> srem = take;
+        uint32_t lastS = 0;
+        bool first = true;
+        while (!srem.empty()) {
+            uint32_t S = 0;
+            uint64_t sblocked = 0;
+            std::vector<int> stake, sdefer;
+            for (int id : srem) {
+                const GOp &op = g[id];
+                const uint64_t tq = op.touched();
+                if (tq & sblocked) {
+                    sblocked |= tq;
+                    sdefer.push_back(id);
+                    continue;
+                }
+                uint32_t need = 0;
+                for (uint64_t m = op.need(); m; m &= m - 1) need |= 1u << local(__builtin_ctzll(m));
+                if (popc(S | need) <= kRegBits && !(first && (need & lowc))) {
+                    S |= need;
+                    stake.push_back(id);
+                } else {
+                    sblocked |= tq;
+                    sdefer.push_back(id);
+                }
+            }
+            srem.swap(sdefer);
+            S = pad_registers(S, t, lowc, first);
+            emit_stage(S, stake);
+            lastS = S;
+            first = false;
         }
+        // The last stage scatters the tile: if its register bits sit on the
+        // coalesced positions, one more (op-free) stage does the scatter.
+        if (lastS & lowc) emit_stage(pad_registers(0, t, lowc, true), {});
         ps.nstages = int(P.stages.size()) - ps.stage_begin;
         P.passes.push_back(ps);
     }
@@ -373,7 +717,7 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 {
-    FusedConfig f;
+    FusedConfig f{};
     const int tmax = fused::kMaxTile;  // 256 threads x 16 amplitudes
     f.t = tmax;
     f.low = c64 ? 5 : 4;  // 256-byte contiguous rows
@@ -392,6 +736,7 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
     EV_REQUIRE(f.low >= 1 && f.low <= 6, EV_E_ARG, "low_qubits must be in [1, 6]");
     f.t = std::min(f.t, n);
     f.low = std::min(f.low, f.t - 1);
+    f.coalesce = c64 ? 4 : 3;  // 128-byte runs
     return f;
 }
 
@@ -458,7 +803,7 @@ void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st
     if (low.ops.empty()) return;
     const double t0 = now_ms();
     const float budget = (cfg.t >= s->n) ? 1e30f : cfg.budget;  // one tile holds the state: one pass
-    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget);
+    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce);
     st.plan_ms += now_ms() - t0;
     run_plan(s, P, st, prof);
     low.ops.clear();
@@ -521,7 +866,7 @@ Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg)
     }
     low.flush_all();
     const float budget = (cfg.t >= n) ? 1e30f : cfg.budget;
-    return plan(low.ops, low.pool, n, cfg.t, cfg.low, budget);
+    return plan(low.ops, low.pool, n, cfg.t, cfg.low, budget, cfg.coalesce);
 }
 
 PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)

@@ -15,11 +15,13 @@ namespace ev {
 void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, const ev_run_options &opt,
                      RunStats &st, Profiler &prof);
 
-// Fused pass configuration: tile width t, low qubits always in the tile, and
-// the per-pass compute budget in units of one dense 2x2 over the tile.
+// Fused pass configuration: tile width t, low qubits always in the tile, the
+// per-pass compute budget in units of one dense 2x2 over the tile, and the
+// number of low qubits whose amplitudes fill one 128-byte run.
 struct FusedConfig {
     int t, low;
     float budget;
+    int coalesce;
 };
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt);
 

@@ -238,9 +238,7 @@ void step(ev_server *s)
     hipError_t e = hipMemcpyAsync(s->d_ent, s->h_ent, na * sizeof(fused::BatchEntry), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipEventRecord(a, s->stream);
     if (e == hipSuccess) {
-        const unsigned per_cu = unsigned(std::max<size_t>(1, (160u << 10) / ((size_t(1) << max_t) * s->amp_bytes)));
-        fused::launch_batch(s->c64, static_cast<const fused::BatchEntry *>(s->d_ent), int(na), tiles, max_t,
-                            256u * std::min(per_cu, 8u), s->stream);
+        fused::launch_batch(s->c64, static_cast<const fused::BatchEntry *>(s->d_ent), int(na), tiles, max_t, s->stream);
         e = hipEventRecord(b, s->stream);
     }
     if (e == hipSuccess) e = hipEventSynchronize(b);

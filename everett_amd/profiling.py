@@ -1,0 +1,154 @@
+"""Summaries of rocprofv3 results (the run_results.db SQLite files it writes).
+
+Run as a script (it needs only the standard library):
+
+  python everett_amd/profiling.py trace --db DB --command CMD --out FILE
+      per-kernel dispatch count, total and average duration from a
+      `rocprofv3 --kernel-trace --stats` run.
+
+  python everett_amd/profiling.py traffic --fetch DB --write DB --kernel NAME
+      --copy-bytes B --algorithmic B --key KEY --json FILE --out FILE
+      HBM bytes per launch of one kernel from two `rocprofv3 --pmc` runs
+      (FETCH_SIZE, then WRITE_SIZE).  Both counters are scaled by what they
+      read for the streaming-copy kernel (k_copy) in the same run, whose
+      bytes read and written per launch are known, so the result is in bytes
+      of the same kind as the copy's.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import statistics
+import sys
+from pathlib import Path
+
+
+def short_name(name: str) -> str:
+    """ev::fused::(anonymous namespace)::k_fused_pass<double>(...) -> k_fused_pass<double>."""
+    s = name.replace("(anonymous namespace)::", "").split("(", 1)[0]
+    depth, cut = 0, 0
+    for i, ch in enumerate(s):  # drop namespace qualifiers outside template arguments
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == ":" and depth == 0 and s[i + 1:i + 2] == ":":
+            cut = i + 2
+    return s[cut:] or name
+
+
+def _kernel_rows(db: Path):
+    con = sqlite3.connect(str(db))
+    rows = con.execute("SELECT name, start, end FROM kernels").fetchall()
+    con.close()
+    per: dict[str, list[float]] = {}
+    for name, s, e in rows:
+        per.setdefault(name, []).append((e - s) * 1e-3)  # ns -> us
+    return per
+
+
+def trace(args) -> int:
+    per = _kernel_rows(Path(args.db))
+    total = sum(sum(v) for v in per.values())
+    lines = [f"# {args.title}", "", f"command: {args.command}", f"source: rocprofv3 kernel trace ({args.db})", ""]
+    lines.append(f"{'kernel':<44} {'calls':>7} {'total ms':>11} {'avg us':>11} {'min us':>10} {'max us':>10} {'%':>6}")
+    for name, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{short_name(name)[:44]:<44} {len(d):>7} {sum(d) * 1e-3:>11.3f} {statistics.fmean(d):>11.2f} "
+                     f"{min(d):>10.2f} {max(d):>10.2f} {100 * sum(d) / total:>6.2f}")
+    text = "\n".join(lines) + "\n"
+    Path(args.out).write_text(text)
+    print(text, end="")
+    return 0
+
+
+def _counter(db: Path, counter: str) -> dict[str, list[float]]:
+    con = sqlite3.connect(str(db))
+    rows = con.execute("SELECT dispatch_id, kernel_name, value FROM counters_collection WHERE counter_name = ?",
+                       (counter,)).fetchall()
+    con.close()
+    per: dict[str, list[float]] = {}
+    for _, name, value in rows:
+        per.setdefault(name, []).append(float(value))
+    return per
+
+
+def _mean_for(per: dict[str, list[float]], pattern: str) -> tuple[float, int]:
+    vals = [v for name, vs in per.items() if pattern in name for v in vs]
+    if not vals:
+        raise SystemExit(f"no dispatches of a kernel matching {pattern!r}")
+    return statistics.fmean(vals), len(vals)
+
+
+def traffic(args) -> int:
+    fetch = _counter(Path(args.fetch), "FETCH_SIZE")
+    write = _counter(Path(args.write), "WRITE_SIZE")
+    k_fetch, nf = _mean_for(fetch, args.kernel)
+    k_write, nw = _mean_for(write, args.kernel)
+    c_fetch, _ = _mean_for(fetch, args.copy_kernel)
+    c_write, _ = _mean_for(write, args.copy_kernel)
+    f_fetch = args.copy_bytes / (c_fetch * 1024.0)
+    f_write = args.copy_bytes / (c_write * 1024.0)
+    fetched = k_fetch * 1024.0 * f_fetch
+    written = k_write * 1024.0 * f_write
+    total = fetched + written
+    entry = {
+        "hbm_bytes_per_launch": total,
+        "fetch_bytes_per_launch": fetched,
+        "write_bytes_per_launch": written,
+        "algorithmic_bytes_per_launch": args.algorithmic,
+        "traffic_over_algorithmic": total / args.algorithmic,
+        "fetch_scale": f_fetch,
+        "write_scale": f_write,
+        "dispatches": {"fetch": nf, "write": nw},
+        "commands": {"fetch": args.fetch_command, "write": args.write_command},
+    }
+    jf = Path(args.json)
+    table = json.loads(jf.read_text()) if jf.exists() else {}
+    table[args.key] = entry
+    jf.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    lines = [
+        f"# {args.title}", "",
+        f"fetch run: {args.fetch_command}",
+        f"write run: {args.write_command}", "",
+        f"kernel matched: {args.kernel} ({nf} dispatches with FETCH_SIZE, {nw} with WRITE_SIZE)",
+        f"FETCH_SIZE mean {k_fetch:.6g} KiB/dispatch; copy kernel {c_fetch:.6g} KiB for {args.copy_bytes} B read "
+        f"-> scale {f_fetch:.4f}",
+        f"WRITE_SIZE mean {k_write:.6g} KiB/dispatch; copy kernel {c_write:.6g} KiB for {args.copy_bytes} B written "
+        f"-> scale {f_write:.4f}",
+        f"HBM bytes per launch: read {fetched:.6g} + write {written:.6g} = {total:.6g}",
+        f"algorithmic bytes per launch: {args.algorithmic:.6g}  (traffic / algorithmic = {total / args.algorithmic:.4f})",
+    ]
+    text = "\n".join(lines) + "\n"
+    Path(args.out).write_text(text)
+    print(text, end="")
+    return 0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("--db", required=True)
+    t.add_argument("--command", required=True, help="the rocprofv3 command that wrote the db")
+    t.add_argument("--title", default="rocprofv3 kernel trace")
+    t.add_argument("--out", required=True)
+    w = sub.add_parser("traffic")
+    w.add_argument("--fetch", required=True)
+    w.add_argument("--write", required=True)
+    w.add_argument("--fetch-command", required=True)
+    w.add_argument("--write-command", required=True)
+    w.add_argument("--kernel", required=True, help="substring of the kernel name")
+    w.add_argument("--copy-kernel", default="k_copy")
+    w.add_argument("--copy-bytes", type=float, required=True, help="bytes read (= written) per copy launch")
+    w.add_argument("--algorithmic", type=float, required=True, help="algorithmic bytes per launch of the kernel")
+    w.add_argument("--key", required=True)
+    w.add_argument("--json", required=True)
+    w.add_argument("--title", default="HBM traffic per launch")
+    w.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    return trace(args) if args.cmd == "trace" else traffic(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,10 +1,12 @@
 """CPU emulation of the fused pass kernel (everett_amd/csrc/fused_kernels.hip).
 
 Replays a plan from ev_debug_plan with the kernel's exact index arithmetic
-(tile base, load mapping, XOR-swizzled LDS slots, per-stage register
-mapping, conditions) on a numpy state, asserting that every HBM index is in
-range and every LDS mapping is a permutation, so the plan and the kernel's
-indexing are checked on the CPU before a GPU runs them.
+(tile base by zero-bit insertion, the global gather in the first stage's
+register mapping, XOR-swizzled LDS slots between stages, the global scatter
+in the last stage's mapping, conditions) on a numpy state, asserting that
+every HBM index is in range and every mapping is a permutation of the tile,
+so the plan and the kernel's indexing are checked on the CPU before a GPU
+runs them.
 """
 from __future__ import annotations
 
@@ -56,9 +58,27 @@ def _coef_span(kind):
     return {U2: 16}.get(kind, 0)
 
 
-def check_plan(n, passes, stages, ops, coef):
-    """Structural bounds of a plan, independent of any state."""
+def check_plan(n, passes, stages, ops, coef, coalesce=3):
+    """Structural bounds of a plan, independent of any state.
+
+    The first and last stage of a pass must keep their register bits off the
+    `coalesce` lowest local positions (3 for c128, 4 for c64) wherever those
+    are the state's lowest qubits, so the global gather and scatter move
+    128-byte runs."""
     for P in passes:
+        t0 = int(P["t"])
+        low = 0
+        while low < t0 and int(P["q"][low]) == low:
+            low += 1
+        lowc = (1 << min(coalesce, low)) - 1
+        for s in (0, int(P["nstages"]) - 1):
+            st = stages[P["stage_begin"] + s]
+            regpos = 0
+            for j in range(4):
+                gq = int(st["goff"][1 << j]).bit_length() - 1
+                regpos |= 1 << list(P["q"][:t0]).index(gq)
+            assert not (regpos & lowc), "first/last stage registers must leave the coalesced low qubits to lanes"
+            assert [int(x) for x in st["p"][: min(coalesce, low)]] == list(range(min(coalesce, low)))
         t = int(P["t"])
         assert 8 <= t <= 12 and t <= n and P["ncomp"] == n - t
         qs = sorted(list(P["q"][:t]) + list(P["comp"][: n - t]))
@@ -152,6 +172,25 @@ def _apply(o, v, gfix, coef):
             v[:, b] = t
 
 
+def tile_base(T, q):
+    """The kernel's tile base: a zero bit inserted at every tile qubit (ascending)."""
+    b = T
+    for k in q:
+        lo = b & ((1 << k) - 1)
+        b = ((b >> k) << (k + 1)) | lo
+    return b
+
+
+def _lane(st, tid, t, base):
+    lb = np.zeros(len(tid), dtype=np.int64)
+    gfix = np.full(len(tid), base, dtype=np.uint64)
+    for i in range(t - 4):
+        bit = ((tid >> i) & 1)
+        lb |= bit << int(st["p"][i])
+        gfix |= bit.astype(np.uint64) << np.uint64(st["gpos"][i])
+    return lb, gfix
+
+
 def emulate(psi, n, passes, stages, ops, coef):
     """Runs the plan on psi (complex128, length 2^n) in place."""
     for P in passes:
@@ -160,38 +199,35 @@ def emulate(psi, n, passes, stages, ops, coef):
         tid = np.arange(W, dtype=np.int64)
         q = [int(x) for x in P["q"][:t]]
         comp = [int(x) for x in P["comp"][: n - t]]
-        gthr = np.zeros(W, dtype=np.int64)
-        for i in range(t - 4):
-            gthr |= ((tid >> i) & 1) << q[i]
+        nst = int(P["nstages"])
         for T in range(1 << (n - t)):
-            base = 0
+            base = tile_base(T, q)
+            want = 0
             for i, c in enumerate(comp):
-                base |= ((T >> i) & 1) << c
+                want |= ((T >> i) & 1) << c
+            assert base == want, "tile base must deposit the tile index into the non-tile qubits"
             lds = np.full(1 << t, np.nan, dtype=np.complex128)
-            idxs = []
-            for j in range(16):
-                gj = 0
-                for kk in range(4):
-                    if (j >> kk) & 1:
-                        gj |= 1 << q[t - 4 + kk]
-                idx = base | gthr | gj
-                assert idx.max() < (1 << n)
-                lds[swz(j * W + tid)] = psi[idx]
-                idxs.append(idx)
-            assert not np.isnan(lds).any()
-            for s in range(P["nstages"]):
+            tile_idx = None
+            for s in range(nst):
                 st = stages[P["stage_begin"] + s]
-                lb = np.zeros(W, dtype=np.int64)
-                gfix = np.full(W, base, dtype=np.uint64)
-                for i in range(t - 4):
-                    bit = ((tid >> i) & 1).astype(np.uint64)
-                    lb |= ((tid >> i) & 1) << int(st["p"][i])
-                    gfix |= bit << np.uint64(st["gpos"][i])
-                slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
-                v = lds[slots]
+                lb, gfix = _lane(st, tid, t, base)
+                if s == 0:
+                    gidx = (gfix[:, None] | st["goff"][None, :].astype(np.uint64)).astype(np.int64)
+                    assert gidx.max() < (1 << n) and len(np.unique(gidx)) == (1 << t)
+                    tile_idx = set(gidx.ravel().tolist())
+                    v = psi[gidx]
+                else:
+                    slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+                    v = lds[slots]
+                    assert not np.isnan(v).any(), "a stage read an LDS slot nobody wrote"
                 for o in ops[st["op_begin"]: st["op_end"]]:
                     _apply(o, v, gfix, coef)
-                lds[slots] = v
-            for j in range(16):
-                psi[idxs[j]] = lds[swz(j * W + tid)]
+                if s + 1 < nst:
+                    slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+                    lds[:] = np.nan
+                    lds[slots] = v
+                else:
+                    gidx = (gfix[:, None] | st["goff"][None, :].astype(np.uint64)).astype(np.int64)
+                    assert set(gidx.ravel().tolist()) == tile_idx, "scatter must cover the gathered tile"
+                    psi[gidx] = v
     return psi

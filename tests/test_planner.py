@@ -64,7 +64,7 @@ def test_plan_emulation_matches_oracle(name, make, opts):
     c = make()
     n = c.num_qubits()
     passes, stages, ops, coef, t = get_plan(c, **opts)
-    check_plan(n, passes, stages, ops, coef)
+    check_plan(n, passes, stages, ops, coef, coalesce=4 if opts.get("precision") == "c64" else 3)
     rng = np.random.default_rng(n)
     psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
     psi /= np.linalg.norm(psi)
@@ -87,7 +87,7 @@ def test_plan_structure_at_30_qubits():
 def test_plan_structure_c64_30_qubits():
     c = ev.workloads.config1()
     passes, stages, ops, coef, t = get_plan(c, precision="c64", tile_qubits=12)
-    check_plan(30, passes, stages, ops, coef)
+    check_plan(30, passes, stages, ops, coef, coalesce=4)
     assert t == 12
 
 
