@@ -51,7 +51,8 @@ def _kernel_rows(db: Path):
 def trace(args) -> int:
     per = _kernel_rows(Path(args.db))
     total = sum(sum(v) for v in per.values())
-    lines = [f"# {args.title}", "", f"command: {args.command}", f"source: rocprofv3 kernel trace ({args.db})", ""]
+    lines = [f"# {args.title}", "", f"command: {args.command}",
+             f"source: rocprofv3 kernel trace ({Path(args.db).name}), {sum(len(v) for v in per.values())} dispatches", ""]
     lines.append(f"{'kernel':<44} {'calls':>7} {'total ms':>11} {'avg us':>11} {'min us':>10} {'max us':>10} {'%':>6}")
     for name, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
         lines.append(f"{short_name(name)[:44]:<44} {len(d):>7} {sum(d) * 1e-3:>11.3f} {statistics.fmean(d):>11.2f} "
