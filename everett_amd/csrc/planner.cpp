@@ -63,6 +63,14 @@ struct GOp {
         default: return uint64_t(1) << q0;
         }
     }
+    // need() plus, for a singly-controlled op, its control: with both in
+    // registers the pair can fuse with its neighbours into one 4x4.  Taken
+    // where it fits, else the op runs on need() alone.
+    uint64_t want() const
+    {
+        if ((kind == G_CX || kind == G_CU1) && __builtin_popcountll(cmask) == 1) return need() | cmask;
+        return need();
+    }
 };
 
 // Compute cost in units of one dense 2x2 over the tile.
@@ -233,11 +241,10 @@ Mat4 lift(const Mat2 &m, bool hi)
 
 struct Block {
     enum Kind { Single, Pair, Opaque, Dead } kind;
-    int a = -1, b = -1;  // Single: a; Pair: a = MSB, b = LSB of the basis
+    int a = -1, b = -1;        // Single: a; Pair: a = MSB, b = LSB of the basis
     Mat2 m2{};
     Mat4 m4{};
-    int op = -1;  // Opaque: index into g
-    float cost = 0;
+    std::vector<int> members;  // original ops, in program order
 };
 
 class StageFuser {
@@ -257,15 +264,21 @@ class StageFuser {
         add_opaque(id);
     }
 
-    // The fused stage as planner ops; new dense 4x4 and 2x2 coefficients go to pool.
+    // The fused stage as planner ops; new dense 4x4 and 2x2 coefficients go to
+    // pool.  A pair block becomes one dense 4x4 when its member ops cost at
+    // least as much; otherwise its members run as they are.
     std::vector<GOp> finish(std::vector<c128> &pool) const
     {
         std::vector<GOp> out;
         for (const Block &bl : blocks_) {
             switch (bl.kind) {
             case Block::Dead: break;
-            case Block::Opaque: out.push_back(g_[bl.op]); break;
+            case Block::Opaque: out.push_back(g_[bl.members[0]]); break;
             case Block::Single: {
+                if (bl.members.size() == 1) {  // untouched: keep the op as lowered
+                    out.push_back(g_[bl.members[0]]);
+                    break;
+                }
                 if (is_identity2(bl.m2)) break;
                 GOp o{};
                 o.q0 = bl.a;
@@ -282,8 +295,14 @@ class StageFuser {
                 break;
             }
             case Block::Pair: {
-                if (bl.op >= 0) {  // a lone op that nothing joined: keep its cheaper form
-                    out.push_back(g_[bl.op]);
+                float parts = 0;
+                bool dense = false;
+                for (int id : bl.members) {
+                    parts += fused_cost(g_[id].kind);
+                    dense = dense || g_[id].kind == G_U2;
+                }
+                if (!dense && parts < fused_cost(G_U2)) {
+                    for (int id : bl.members) out.push_back(g_[id]);
                     break;
                 }
                 GOp o{};
@@ -362,18 +381,19 @@ class StageFuser {
         const int o = open_of(q);
         if (o >= 0 && blocks_[o].kind == Block::Single) {
             blocks_[o].m2 = gate::compose(m, blocks_[o].m2);
+            blocks_[o].members.push_back(id);
             return;
         }
         if (o >= 0 && blocks_[o].kind == Block::Pair) {
             Block &bl = blocks_[o];
             bl.m4 = mul4(lift(m, q == bl.a), bl.m4);
-            bl.op = -1;
+            bl.members.push_back(id);
             return;
         }
         Block bl{Block::Single};
         bl.a = q;
         bl.m2 = m;
-        bl.cost = fused_cost(g_[id].kind);
+        bl.members.push_back(id);
         open_[q] = int(blocks_.size());
         blocks_.push_back(bl);
     }
@@ -383,36 +403,25 @@ class StageFuser {
         const int oa = open_of(a), ob = open_of(b);
         if (oa >= 0 && oa == ob && blocks_[oa].kind == Block::Pair) {  // same pair: multiply in
             Block &bl = blocks_[oa];
-            Mat4 mm = m;
-            if (bl.a != a) mm = reorder(m);
-            bl.m4 = mul4(mm, bl.m4);
-            bl.op = -1;
+            bl.m4 = mul4(bl.a != a ? reorder(m) : m, bl.m4);
+            bl.members.push_back(id);
             return;
         }
-        const bool sa = oa >= 0 && blocks_[oa].kind == Block::Single;
-        const bool sb = ob >= 0 && blocks_[ob].kind == Block::Single;
-        const float parts = fused_cost(g_[id].kind) + (sa ? blocks_[oa].cost : 0) + (sb ? blocks_[ob].cost : 0);
-        const bool dense = g_[id].kind == G_U2;
+        // A new pair block takes over the open one-qubit blocks of a and b
+        // (nothing touched those qubits since, so they move forward freely).
         Block bl{Block::Pair};
         bl.a = a;
         bl.b = b;
-        bl.cost = fused_cost(G_U2);
-        if (dense || parts >= bl.cost) {
-            Mat4 pre = lift(gate::id(), true);
-            if (sa) {
-                pre = mul4(lift(blocks_[oa].m2, true), pre);
-                blocks_[oa].kind = Block::Dead;
-            }
-            if (sb) {
-                pre = mul4(lift(blocks_[ob].m2, false), pre);
-                blocks_[ob].kind = Block::Dead;
-            }
-            bl.m4 = mul4(m, pre);
-            bl.op = (sa || sb) ? -1 : id;
-        } else {  // not worth a dense 4x4: keep the op, open for both qubits
-            bl.kind = Block::Opaque;
-            bl.op = id;
+        Mat4 pre = lift(gate::id(), true);
+        for (int o : {oa, ob}) {
+            if (o < 0 || blocks_[o].kind != Block::Single) continue;
+            pre = mul4(lift(blocks_[o].m2, blocks_[o].a == a), pre);
+            bl.members.insert(bl.members.end(), blocks_[o].members.begin(), blocks_[o].members.end());
+            blocks_[o].kind = Block::Dead;
         }
+        std::sort(bl.members.begin(), bl.members.end());  // ids are in program order
+        bl.m4 = mul4(m, pre);
+        bl.members.push_back(id);
         open_[a] = open_[b] = int(blocks_.size());
         blocks_.push_back(bl);
     }
@@ -420,7 +429,7 @@ class StageFuser {
     void add_opaque(int id)
     {
         Block bl{Block::Opaque};
-        bl.op = id;
+        bl.members.push_back(id);
         const int idx = int(blocks_.size());
         for (uint64_t t = g_[id].touched(); t; t &= t - 1) open_[__builtin_ctzll(t)] = idx;
         blocks_.push_back(bl);
@@ -591,7 +600,8 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 blocked |= tq;
                 defer.push_back(rem[i]);
             } else {
-                const uint64_t nq = Q | op.need();
+                uint64_t nq = Q | op.want();
+                if (popc(nq) > t) nq = Q | op.need();
                 if (popc(nq) <= t && (take.empty() || cost + op.cost <= budget)) {
                     Q = nq;
                     cost += op.cost;
@@ -686,8 +696,15 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                     sdefer.push_back(id);
                     continue;
                 }
-                uint32_t need = 0;
-                for (uint64_t m = op.need(); m; m &= m - 1) need |= 1u << local(__builtin_ctzll(m));
+                auto lmask = [&](uint64_t m) {
+                    uint32_t r = 0;
+                    for (; m; m &= m - 1)
+                        if ((Q >> __builtin_ctzll(m)) & 1) r |= 1u << local(__builtin_ctzll(m));
+                    return r;
+                };
+                uint32_t need = lmask(op.want());
+                if (popc(S | need) > kRegBits || (first && (need & lowc)) || popc(need) < popc(op.want()))
+                    need = lmask(op.need());
                 if (popc(S | need) <= kRegBits && !(first && (need & lowc))) {
                     S |= need;
                     stake.push_back(id);
