@@ -40,7 +40,12 @@ enum OpKind : uint8_t {
     kDiagT = 4,  // diag(d0, d1) on global qubit tq (not a register bit) where the condition holds
     kU2 = 5,     // dense 4x4 on register bits (b0 = MSB of the gate basis, b1 = LSB), b0 > b1
     kSwap = 6,   // exchange register bits b0 > b1
+    kPhase = 7,  // multiply registers whose bit b0 is set (b0 = 4: every register) by the lane's
+                 // product of b1 conditioned phase factors (see DevOp)
 };
+
+constexpr uint32_t kPhaseAll = 4;     // kPhase b0: the factor applies to all 16 registers
+constexpr uint32_t kMaxPhaseTerms = 8;
 
 enum OpFlags : uint8_t { kD0IsOne = 1 };
 
@@ -49,6 +54,11 @@ enum OpFlags : uint8_t { kD0IsOne = 1 };
 // and diagonal ops carry their coefficients inline (one fetch per op); kU2
 // indexes the pass's coefficient pool.  In c64 runs the inline coefficients
 // are stored as floats in the first half of m.
+//
+// kPhase carries b1 terms: term i multiplies by factor coef[coef + i] (pool)
+// the lanes whose global index has every bit of mask i set, where mask i is
+// the 64-bit pattern stored in m[i] (not converted for c64).  The lane's
+// factor is the product of its terms' factors.
 struct DevOp {
     uint32_t code;
     uint32_t creg;   // condition on the register index j: (j & creg) == creg
@@ -90,6 +100,7 @@ struct Pass {
 inline void inline_coef_to_c64(DevOp *ops, size_t n)
 {
     for (size_t i = 0; i < n; ++i) {
+        if ((ops[i].code & 0xff) == kPhase) continue;  // m holds term masks
         float f[8];
         for (int k = 0; k < 8; ++k) f[k] = float(ops[i].m[k]);
         for (int k = 0; k < 8; ++k) reinterpret_cast<float *>(ops[i].m)[k] = f[k];

@@ -215,11 +215,13 @@ template <class V> struct OpView {
     uint64_t cglob;
     uint32_t coef, tq;
     V m[4];
+    cptr<DevOp> p;
 };
 
 template <class V> __device__ __forceinline__ OpView<V> fetch_op(cptr<DevOp> p)
 {
     OpView<V> o;
+    o.p = p;
     o.code = p->code;
     o.creg = p->creg;
     o.cglob = p->cglob;
@@ -283,6 +285,25 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, Regs<V> &v, uint64_t
         V m[16];
         load_row<V>(coef + op.coef, m);
         EV_CASE_PAIR(op_u2, v, m);
+        break;
+    }
+    case kPhase: {
+        // the lane's factor: product of the factors whose mask it meets
+        const cptr<uint64_t> masks = (cptr<uint64_t>)(op.p->m);
+        const cptr<V> fac = coef + op.coef;
+        using F = decltype(V::x);
+        const cptr<F> ff = (cptr<F>)(fac);
+        V f = pick((gfix & masks[0]) == masks[0], mk<V>(ff[0], ff[1]), mk<V>(1, 0));
+        for (uint32_t i = 1; i < b1; ++i) {
+            const uint64_t mk_i = masks[i];
+            const V g = cmul(f, mk<V>(ff[2 * i], ff[2 * i + 1]));
+            f = pick((gfix & mk_i) == mk_i, g, f);
+        }
+        if (b0 == kPhaseAll) {
+            op_diagt<V>(v, f, 0u);
+        } else {
+            EV_CASE_B_MS(op_diagr, false, v, mk<V>(1, 0), f, true, 0u);
+        }
         break;
     }
     default: EV_CASE_PAIR(op_swap, v); break;

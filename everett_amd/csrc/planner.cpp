@@ -528,6 +528,73 @@ DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const R
     return d;
 }
 
+// Merges a stage's conditioned phase ops.  A diagonal op whose factor on
+// the 0 side is 1 multiplies, on the lanes whose global bits meet its
+// condition, either the registers with one register bit set (kDiagR) or
+// every register (kDiagT, whose target bit joins the condition).  Diagonal
+// ops commute with each other, and ops inside a stage never change a lane's
+// global bits, so such an op moves back to an earlier phase op of the same
+// register bit as long as no op in between targets that bit (kDiagT terms
+// commute with every op of the stage).  Each merged op computes the lane's
+// product of factors once and scales its registers once.
+std::vector<DevOp> merge_phases(const std::vector<DevOp> &in, std::vector<c128> &coef)
+{
+    struct Group {
+        size_t at;  // position in out
+        std::vector<std::pair<uint64_t, c128>> terms;
+        DevOp first;
+    };
+    std::vector<DevOp> out;
+    std::vector<Group> groups;
+    int open[fused::kPhaseAll + 1];
+    std::fill(open, open + fused::kPhaseAll + 1, -1);
+    for (const DevOp &d : in) {
+        const uint32_t kind = d.code & 0xff, b0 = (d.code >> 8) & 0xff, b1 = (d.code >> 16) & 0xff;
+        const bool one = ((d.code >> 24) & fused::kD0IsOne) != 0;
+        int grp = -1;
+        uint64_t mask = 0;
+        if (kind == fused::kDiagR && one && d.creg == 0) {
+            grp = int(b0);
+            mask = d.cglob;
+        } else if (kind == fused::kDiagT && one && d.creg == 0) {
+            grp = int(fused::kPhaseAll);
+            mask = d.cglob | (uint64_t(1) << d.tq);
+        }
+        if (grp >= 0) {
+            const c128 f{d.m[2], d.m[3]};
+            if (open[grp] < 0 || groups[open[grp]].terms.size() == fused::kMaxPhaseTerms) {
+                open[grp] = int(groups.size());
+                groups.push_back({out.size(), {}, d});
+                out.push_back(d);  // placeholder, rewritten below
+            }
+            groups[open[grp]].terms.push_back({mask, f});
+            continue;
+        }
+        switch (kind) {  // an op that targets a register bit closes its phase group
+        case fused::kU1:
+        case fused::kCU1:
+        case fused::kCX: open[b0] = -1; break;
+        case fused::kU2:
+        case fused::kSwap: open[b0] = open[b1] = -1; break;
+        default: break;
+        }
+        out.push_back(d);
+    }
+    for (const Group &gr : groups) {
+        if (gr.terms.size() == 1) continue;  // the original op stays
+        DevOp m{};
+        const uint32_t b0 = ((gr.first.code & 0xff) == fused::kDiagT) ? fused::kPhaseAll : (gr.first.code >> 8) & 0xff;
+        m.code = fused::op_code(fused::kPhase, b0, uint32_t(gr.terms.size()), 0);
+        m.coef = uint32_t(coef.size());
+        for (size_t i = 0; i < gr.terms.size(); ++i) {
+            std::memcpy(&m.m[i], &gr.terms[i].first, sizeof(uint64_t));
+            coef.push_back(gr.terms[i].second);
+        }
+        out[gr.at] = m;
+    }
+    return out;
+}
+
 // Register positions for a stage that needs `S`: padded to 4 with free
 // positions, never from `avoid` when the stage must stay coalesced, and
 // otherwise scored so that the 16 lanes of an LDS quarter-wave fall on
@@ -670,7 +737,9 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 st.roff[j] = fused::swz(off);
                 st.goff[j] = goff;
             }
-            for (const GOp &op : fused) P.devops.push_back(lower_op(op, pool, S, regbit, P.coef));
+            std::vector<DevOp> lowered;
+            for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef));
+            for (const DevOp &d : merge_phases(lowered, P.coef)) P.devops.push_back(d);
             st.op_end = int(P.devops.size());
             P.stages.push_back(st);
         };

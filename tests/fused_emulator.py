@@ -29,7 +29,9 @@ def decode(o):
     c = int(o["code"])
     return c & 0xFF, (c >> 8) & 0xFF, (c >> 16) & 0xFF, c >> 24
 
-U1, CU1, CX, DIAGR, DIAGT, U2, SWAP = range(7)
+U1, CU1, CX, DIAGR, DIAGT, U2, SWAP, PHASE = range(8)
+PHASE_ALL = 4
+MAX_PHASE_TERMS = 8
 
 
 def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low_qubits=0):
@@ -54,8 +56,13 @@ def swz(l):
     return l ^ (((l >> 4) ^ (l >> 8) ^ (l >> 12)) & 15)
 
 
-def _coef_span(kind):
-    return {U2: 16}.get(kind, 0)
+def _coef_span(kind, b1=0):
+    return {U2: 16, PHASE: b1}.get(kind, 0)
+
+
+def phase_masks(o, k):
+    """The k term masks of a PHASE op (64-bit patterns stored in m)."""
+    return np.frombuffer(o["m"].tobytes(), dtype="<u8")[:k]
 
 
 def check_plan(n, passes, stages, ops, coef, coalesce=3):
@@ -105,10 +112,16 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
                 regq.add(gj.bit_length() - 1)
             for o in ops[st["op_begin"]: st["op_end"]]:
                 k, b0, b1, _ = decode(o)
-                assert k <= SWAP and b0 < 4 and b1 < 4 and o["creg"] < 16
+                assert k <= PHASE and o["creg"] < 16
+                if k == PHASE:
+                    assert b0 <= PHASE_ALL and 2 <= b1 <= MAX_PHASE_TERMS and o["creg"] == 0 and o["cglob"] == 0
+                    for mk in phase_masks(o, b1):
+                        assert int(mk) < (1 << n) and all(not (int(mk) >> q) & 1 for q in regq)
+                else:
+                    assert b0 < 4 and b1 < 4
                 if k in (U2, SWAP):
                     assert b0 > b1
-                assert o["coef"] + _coef_span(k) <= len(coef)
+                assert o["coef"] + _coef_span(k, b1) <= len(coef)
                 cg = int(o["cglob"])
                 assert cg < (1 << n) and all(not (cg >> q) & 1 for q in regq)
                 if k == DIAGT:
@@ -154,6 +167,14 @@ def _apply(o, v, gfix, coef):
         for j in range(16):
             if condj[j]:
                 v[:, j] = np.where(tok, d * v[:, j], v[:, j])
+    elif k == PHASE:
+        f = np.ones(len(gfix), dtype=np.complex128)
+        for i, mk in enumerate(phase_masks(o, BL)):
+            mk = np.uint64(mk)
+            f = np.where((gfix & mk) == mk, f * coef[int(o["coef"]) + i], f)
+        for j in range(16):
+            if B == PHASE_ALL or j & (1 << B):
+                v[:, j] = f * v[:, j]
     elif k == U2:
         M = m[:16].reshape(4, 4)
         for j in range(16):

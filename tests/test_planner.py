@@ -97,3 +97,18 @@ def test_tile_option_bounds():
         get_plan(c, tile_qubits=13)  # 2^13 amplitudes need 512 threads, above the kernel's launch bound
     with pytest.raises(ev.InvalidArgument):
         get_plan(c, tile_qubits=7)
+
+
+def test_controlled_phases_merge_into_phase_ops():
+    # qft(12): 66 controlled phases; within a stage those sharing a register
+    # bit (or acting on thread bits only) become one lane-factor op each
+    from tests.fused_emulator import PHASE, PHASE_ALL, decode
+    c = ev.algorithms.qft(12)
+    passes, stages, ops, coef, t = get_plan(c, tile_qubits=10)
+    kinds = [decode(o)[0] for o in ops]
+    assert kinds.count(PHASE) > 0
+    assert sum(decode(o)[2] for o in ops if decode(o)[0] == PHASE) > 2 * kinds.count(PHASE)
+    assert len(ops) < 66
+    c1 = ev.workloads.config1(10, 6)
+    _, _, ops1, _, _ = get_plan(c1)
+    assert any(decode(o)[0] == PHASE and decode(o)[1] == PHASE_ALL for o in ops1)
