@@ -76,14 +76,23 @@ EV_HOST_DEVICE inline uint32_t op_code(uint32_t kind, uint32_t b0, uint32_t b1, 
 }
 
 // All fields are dwords (or wider) so the kernel reads them with scalar loads.
+//
+// relabel != 0: CNOTs and SWAPs between tile qubits that the planner moved to
+// the exchange before this stage are applied by the exchange itself.  They
+// permute local indices linearly over GF(2), so the amplitude this stage
+// wants at local index l sits in the LDS slot of A(l), for the composed
+// inverse permutation A; the slot is the XOR of rimg[i] over the thread's set
+// id bits and rroff[j] (swizzled images of the stage's bits under A).
 struct Stage {
     int32_t op_begin, op_end;
     int32_t np;               // thread-id bits (t - 4)
-    int32_t pad;
+    int32_t relabel;          // the LDS read goes through rimg / rroff
     uint32_t roff[kRegs];     // swizzled LDS slot offset of register j
     uint64_t goff[kRegs];     // global index offset of register j
     int32_t p[kMaxTile];      // thread-id bit i -> local position p[i]
     int32_t gpos[kMaxTile];   // thread-id bit i -> global qubit
+    uint32_t rimg[kMaxTile];  // relabel: swizzled A(1 << p[i])
+    uint32_t rroff[kRegs];    // relabel: swizzled A(register offset of j)
 };
 
 struct Pass {

@@ -408,8 +408,14 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
         const cptr<Stage> st = tr.stages + s;
         const Lane ln = lane_of(st, tid, tr.base);
         if (s > 0 && active) {
+            uint32_t rb = ln.sb;
+            if (st->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before this stage
+                rb = 0;
+                for (int i = 0; i < st->np; ++i)
+                    if ((tid >> i) & 1u) rb ^= st->rimg[i];
+            }
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) v[j] = lds[ln.sb ^ st->roff[j]];
+            for (int j = 0; j < kRegs; ++j) v[j] = lds[rb ^ st->rroff[j]];
         }
         run_stage_ops<V>(st, v, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {

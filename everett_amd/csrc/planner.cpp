@@ -707,7 +707,9 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             if (!((S >> l) & 1)) return -1;
             return rank_in(S, l);
         };
-        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake) {
+        // relabel: the composed inverse permutation A (A[k] = image of local
+        // bit k) that this stage's LDS read applies, or null
+        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake, const std::vector<uint32_t> *relabel) {
             uint64_t regq = 0;  // global qubits held in registers
             for (int l = 0; l < t; ++l)
                 if ((S >> l) & 1) regq |= uint64_t(1) << ps.q[l];
@@ -737,6 +739,20 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 st.roff[j] = fused::swz(off);
                 st.goff[j] = goff;
             }
+            st.relabel = relabel ? 1 : 0;
+            auto image = [&](uint32_t x) {  // A(x), x a local index
+                if (!relabel) return x;
+                uint32_t r = 0;
+                for (; x; x &= x - 1) r ^= (*relabel)[__builtin_ctz(x)];
+                return r;
+            };
+            for (int i = 0; i < np; ++i) st.rimg[i] = fused::swz(image(1u << st.p[i]));
+            for (int j = 0; j < kRegs; ++j) {
+                uint32_t off = 0;
+                for (int b = 0; b < kRegBits; ++b)
+                    if ((j >> b) & 1) off |= 1u << sp[b];
+                st.rroff[j] = fused::swz(image(off));
+            }
             std::vector<DevOp> lowered;
             for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef));
             for (const DevOp &d : merge_phases(lowered, P.coef)) P.devops.push_back(d);
@@ -750,13 +766,24 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         int contiguous = 0;  // leading tile positions that are the state's lowest qubits
         while (contiguous < t && ps.q[contiguous] == contiguous) ++contiguous;
         const uint32_t lowc = (1u << std::min(coalesce, contiguous)) - 1;
+        //
+        // A CNOT or SWAP between tile qubits whose qubits do not fit the
+        // stage's registers is not run as an op: it is a linear relabelling
+        // of local indices, which the exchange after the stage applies by
+        // reading through its inverse (relabel tables of the next stage).
+        // Later ops on its qubits then wait for the next stage.
         std::vector<int> srem = take;
         uint32_t lastS = 0;
         bool first = true;
+        std::vector<uint32_t> ident(t), a_in, a_out;
+        for (int k2 = 0; k2 < t; ++k2) ident[k2] = 1u << k2;
+        bool in_pending = false;
         while (!srem.empty()) {
             uint32_t S = 0;
             uint64_t sblocked = 0;
             std::vector<int> stake, sdefer;
+            a_out = ident;
+            bool out_pending = false;
             for (int id : srem) {
                 const GOp &op = g[id];
                 const uint64_t tq = op.touched();
@@ -771,9 +798,25 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                         if ((Q >> __builtin_ctzll(m)) & 1) r |= 1u << local(__builtin_ctzll(m));
                     return r;
                 };
-                uint32_t need = lmask(op.want());
-                if (popc(S | need) > kRegBits || (first && (need & lowc)) || popc(need) < popc(op.want()))
-                    need = lmask(op.need());
+                const uint32_t wantm = lmask(op.want());
+                const bool fits_want = popc(S | wantm) <= kRegBits && !(first && (wantm & lowc)) &&
+                                       popc(wantm) == popc(op.want());
+                const bool perm = ((op.kind == G_CX && __builtin_popcountll(op.cmask) == 1) || op.kind == G_SWAP) &&
+                                  (tq & ~Q) == 0;
+                if (perm && !fits_want) {
+                    // compose A <- A o pi^-1 (pi is its own inverse)
+                    if (op.kind == G_CX) {
+                        const int lc = local(__builtin_ctzll(op.cmask)), lt = local(op.q0);
+                        a_out[lc] ^= a_out[lt];
+                    } else {
+                        std::swap(a_out[local(op.q0)], a_out[local(op.q1)]);
+                    }
+                    out_pending = true;
+                    sblocked |= tq;
+                    continue;
+                }
+                uint32_t need = wantm;
+                if (!fits_want) need = lmask(op.need());
                 if (popc(S | need) <= kRegBits && !(first && (need & lowc))) {
                     S |= need;
                     stake.push_back(id);
@@ -784,13 +827,16 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             }
             srem.swap(sdefer);
             S = pad_registers(S, t, lowc, first);
-            emit_stage(S, stake);
+            emit_stage(S, stake, in_pending ? &a_in : nullptr);
+            a_in = a_out;
+            in_pending = out_pending;
             lastS = S;
             first = false;
         }
         // The last stage scatters the tile: if its register bits sit on the
-        // coalesced positions, one more (op-free) stage does the scatter.
-        if (lastS & lowc) emit_stage(pad_registers(0, t, lowc, true), {});
+        // coalesced positions, or relabellings follow it, one more (op-free)
+        // stage does the scatter.
+        if ((lastS & lowc) || in_pending) emit_stage(pad_registers(0, t, lowc, true), {}, in_pending ? &a_in : nullptr);
         ps.nstages = int(P.stages.size()) - ps.stage_begin;
         P.passes.push_back(ps);
     }

@@ -19,8 +19,9 @@ from everett_amd.errors import check
 
 PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"), ("ncomp", "<i4"),
                        ("q", "<i4", (16,)), ("comp", "<i4", (40,))])
-STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("pad", "<i4"),
-                        ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,))])
+STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("relabel", "<i4"),
+                        ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,)),
+                        ("rimg", "<u4", (12,)), ("rroff", "<u4", (16,))])
 OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4"),
                      ("pad", "<u4", (2,)), ("m", "<c16", (4,))])
 
@@ -105,6 +106,13 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
             slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
             assert slots.min() >= 0 and slots.max() < (1 << t)
             assert len(np.unique(slots)) == (1 << t), "stage mapping must be a permutation of the tile"
+            rslots = _read_slots(st, tid, t, lb)
+            assert rslots.min() >= 0 and rslots.max() < (1 << t)
+            assert len(np.unique(rslots)) == (1 << t), "relabelled read must be a permutation of the tile"
+            if s == 0:
+                assert not int(st["relabel"]), "the first stage reads HBM, not a relabelled exchange"
+            if not int(st["relabel"]):
+                assert (st["rroff"] == st["roff"]).all()
             regq = set()
             for j in range(4):
                 gj = int(st["goff"][1 << j])
@@ -202,6 +210,16 @@ def tile_base(T, q):
     return b
 
 
+def _read_slots(st, tid, t, lb):
+    """LDS slots a stage reads: its own mapping, or through the relabelling tables."""
+    if not int(st["relabel"]):
+        return swz(lb)[:, None] ^ st["rroff"][None, :].astype(np.int64)
+    rb = np.zeros(len(tid), dtype=np.int64)
+    for i in range(t - 4):
+        rb ^= ((tid >> i) & 1) * int(st["rimg"][i])
+    return rb[:, None] ^ st["rroff"][None, :].astype(np.int64)
+
+
 def _lane(st, tid, t, base):
     lb = np.zeros(len(tid), dtype=np.int64)
     gfix = np.full(len(tid), base, dtype=np.uint64)
@@ -238,8 +256,7 @@ def emulate(psi, n, passes, stages, ops, coef):
                     tile_idx = set(gidx.ravel().tolist())
                     v = psi[gidx]
                 else:
-                    slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
-                    v = lds[slots]
+                    v = lds[_read_slots(st, tid, t, lb)]
                     assert not np.isnan(v).any(), "a stage read an LDS slot nobody wrote"
                 for o in ops[st["op_begin"]: st["op_end"]]:
                     _apply(o, v, gfix, coef)

@@ -109,6 +109,20 @@ def test_controlled_phases_merge_into_phase_ops():
     assert kinds.count(PHASE) > 0
     assert sum(decode(o)[2] for o in ops if decode(o)[0] == PHASE) > 2 * kinds.count(PHASE)
     assert len(ops) < 66
-    c1 = ev.workloads.config1(10, 6)
-    _, _, ops1, _, _ = get_plan(c1)
+    _, _, ops1, _, _ = get_plan(mixed_circuit(10, 120, 1), tile_qubits=8)
     assert any(decode(o)[0] == PHASE and decode(o)[1] == PHASE_ALL for o in ops1)
+
+
+def test_tile_cnots_become_exchange_relabellings():
+    # a CNOT between tile qubits that does not fit a stage's registers is
+    # applied by the next exchange's read tables, never as a lane-conditioned op
+    from tests.fused_emulator import CX, decode
+    c = ev.workloads.config0(12, 8)
+    passes, stages, ops, coef, t = get_plan(c, tile_qubits=9)
+    assert sum(int(st["relabel"]) for st in stages) > 0
+    for P in passes:
+        tile = set(int(q) for q in P["q"][: int(P["t"])])
+        for st in stages[int(P["stage_begin"]): int(P["stage_begin"]) + int(P["nstages"])]:
+            for o in ops[st["op_begin"]: st["op_end"]]:
+                if decode(o)[0] == CX and int(o["cglob"]):
+                    assert not any((int(o["cglob"]) >> q) & 1 for q in tile)
