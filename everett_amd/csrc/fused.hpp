@@ -31,6 +31,7 @@ constexpr int kRegs = 1 << kRegBits;  // amplitudes per thread per stage
 constexpr int kMaxTile = 12;          // 2^12 amplitudes: 256 threads x 16 (64 KiB of LDS at c128, 32 KiB at c64)
 constexpr int kMaxThreads = 1 << (kMaxTile - kRegBits);
 constexpr int kMinTile = 8;
+constexpr int kMaxAffine = 8;         // relabelling terms conditioned on tile-index bits, per stage
 
 enum OpKind : uint8_t {
     kU1 = 0,     // dense 2x2 on register bit b0
@@ -82,7 +83,10 @@ EV_HOST_DEVICE inline uint32_t op_code(uint32_t kind, uint32_t b0, uint32_t b1, 
 // permute local indices linearly over GF(2), so the amplitude this stage
 // wants at local index l sits in the LDS slot of A(l), for the composed
 // inverse permutation A; the slot is the XOR of rimg[i] over the thread's set
-// id bits and rroff[j] (swizzled images of the stage's bits under A).
+// id bits and rroff[j] (swizzled images of the stage's bits under A).  A
+// CNOT controlled by a qubit outside the tile flips its target only in the
+// tiles whose index has that bit set, so A may also XOR the constant
+// aoff[k] into every slot of the tiles whose global base has bit abit[k].
 struct Stage {
     int32_t op_begin, op_end;
     int32_t np;               // thread-id bits (t - 4)
@@ -93,6 +97,10 @@ struct Stage {
     int32_t gpos[kMaxTile];   // thread-id bit i -> global qubit
     uint32_t rimg[kMaxTile];  // relabel: swizzled A(1 << p[i])
     uint32_t rroff[kRegs];    // relabel: swizzled A(register offset of j)
+    int32_t naff;             // relabel: affine terms
+    uint32_t abit[kMaxAffine];
+    uint32_t aoff[kMaxAffine];
+    int32_t pad2;
 };
 
 struct Pass {

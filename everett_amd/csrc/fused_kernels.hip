@@ -413,12 +413,18 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
                 rb = 0;
                 for (int i = 0; i < st->np; ++i)
                     if ((tid >> i) & 1u) rb ^= st->rimg[i];
+                for (int k = 0; k < st->naff; ++k)  // CNOTs controlled by the tile index
+                    if ((tr.base >> st->abit[k]) & 1u) rb ^= st->aoff[k];
             }
 #pragma unroll
             for (int j = 0; j < kRegs; ++j) v[j] = lds[rb ^ st->rroff[j]];
         }
         run_stage_ops<V>(st, v, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
+            // A relabelled read took other threads' slots: every thread must
+            // be done reading before any overwrites them.  (Otherwise each
+            // thread writes exactly the slots it read.)
+            if (s > 0 && st->relabel) lds_barrier();
             if (active) {
 #pragma unroll
                 for (int j = 0; j < kRegs; ++j) lds[ln.sb ^ st->roff[j]] = v[j];

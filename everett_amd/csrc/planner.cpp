@@ -270,7 +270,7 @@ class StageFuser {
     std::vector<GOp> finish(std::vector<c128> &pool) const
     {
         std::vector<GOp> out;
-        for (const Block &bl : blocks_) {
+        for (const Block &bl : kron_singles()) {
             switch (bl.kind) {
             case Block::Dead: break;
             case Block::Opaque: out.push_back(g_[bl.members[0]]); break;
@@ -320,6 +320,48 @@ class StageFuser {
     }
 
   private:
+    bool dense_single(const Block &bl) const
+    {
+        return bl.kind == Block::Single && !is_identity2(bl.m2) && classify1(bl.m2.data()) == K1::Dense;
+    }
+
+    uint64_t block_touches(const Block &bl) const
+    {
+        switch (bl.kind) {
+        case Block::Single: return uint64_t(1) << bl.a;
+        case Block::Pair: return (uint64_t(1) << bl.a) | (uint64_t(1) << bl.b);
+        case Block::Opaque: return g_[bl.members[0]].touched();
+        default: return 0;
+        }
+    }
+
+    // Two dense one-qubit blocks on different qubits cost what one dense 4x4
+    // costs (their Kronecker product), so pairing them saves a whole op.  The
+    // later block moves back to the earlier one when no block in between
+    // touches its qubit.
+    std::vector<Block> kron_singles() const
+    {
+        std::vector<Block> bl = blocks_;
+        for (size_t i = 0; i < bl.size(); ++i) {
+            if (!dense_single(bl[i])) continue;
+            uint64_t between = 0;
+            for (size_t k = i + 1; k < bl.size(); ++k) {
+                if (dense_single(bl[k]) && bl[k].a != bl[i].a && !((between >> bl[k].a) & 1)) {
+                    Block &p = bl[i];
+                    p.kind = Block::Pair;
+                    p.b = bl[k].a;  // p.a stays the basis MSB
+                    p.m4 = mul4(lift(p.m2, true), lift(bl[k].m2, false));
+                    p.members.insert(p.members.end(), bl[k].members.begin(), bl[k].members.end());
+                    std::sort(p.members.begin(), p.members.end());
+                    bl[k].kind = Block::Dead;
+                    break;
+                }
+                between |= block_touches(bl[k]);
+            }
+        }
+        return bl;
+    }
+
     static bool is_identity2(const Mat2 &m)
     {
         return ceq(m[0], c128{1, 0}) && ceq(m[1], c128{0, 0}) && ceq(m[2], c128{0, 0}) && ceq(m[3], c128{1, 0});
@@ -707,9 +749,14 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             if (!((S >> l) & 1)) return -1;
             return rank_in(S, l);
         };
-        // relabel: the composed inverse permutation A (A[k] = image of local
-        // bit k) that this stage's LDS read applies, or null
-        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake, const std::vector<uint32_t> *relabel) {
+        // relabel: the composed inverse permutation A (linear part: image of
+        // each local bit; affine part: (global bit, offset) terms) that this
+        // stage's LDS read applies, or null
+        struct Relabel {
+            std::vector<uint32_t> lin;
+            std::vector<std::pair<int, uint32_t>> aff;
+        };
+        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake, const Relabel *relabel) {
             uint64_t regq = 0;  // global qubits held in registers
             for (int l = 0; l < t; ++l)
                 if ((S >> l) & 1) regq |= uint64_t(1) << ps.q[l];
@@ -740,12 +787,19 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 st.goff[j] = goff;
             }
             st.relabel = relabel ? 1 : 0;
-            auto image = [&](uint32_t x) {  // A(x), x a local index
+            auto image = [&](uint32_t x) {  // linear part of A at x, a local index
                 if (!relabel) return x;
                 uint32_t r = 0;
-                for (; x; x &= x - 1) r ^= (*relabel)[__builtin_ctz(x)];
+                for (; x; x &= x - 1) r ^= relabel->lin[__builtin_ctz(x)];
                 return r;
             };
+            if (relabel) {
+                st.naff = int(relabel->aff.size());
+                for (size_t k2 = 0; k2 < relabel->aff.size(); ++k2) {
+                    st.abit[k2] = uint32_t(relabel->aff[k2].first);
+                    st.aoff[k2] = fused::swz(relabel->aff[k2].second);
+                }
+            }
             for (int i = 0; i < np; ++i) st.rimg[i] = fused::swz(image(1u << st.p[i]));
             for (int j = 0; j < kRegs; ++j) {
                 uint32_t off = 0;
@@ -775,8 +829,9 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         std::vector<int> srem = take;
         uint32_t lastS = 0;
         bool first = true;
-        std::vector<uint32_t> ident(t), a_in, a_out;
-        for (int k2 = 0; k2 < t; ++k2) ident[k2] = 1u << k2;
+        Relabel ident, a_in, a_out;
+        ident.lin.resize(t);
+        for (int k2 = 0; k2 < t; ++k2) ident.lin[k2] = 1u << k2;
         bool in_pending = false;
         while (!srem.empty()) {
             uint32_t S = 0;
@@ -801,18 +856,38 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 const uint32_t wantm = lmask(op.want());
                 const bool fits_want = popc(S | wantm) <= kRegBits && !(first && (wantm & lowc)) &&
                                        popc(wantm) == popc(op.want());
-                const bool perm = ((op.kind == G_CX && __builtin_popcountll(op.cmask) == 1) || op.kind == G_SWAP) &&
-                                  (tq & ~Q) == 0;
-                if (perm && !fits_want) {
+                const bool cx1 = op.kind == G_CX && __builtin_popcountll(op.cmask) == 1;
+                const bool perm = (cx1 || op.kind == G_SWAP) && (tq & ~Q) == 0;
+                // a CNOT controlled from outside the tile: affine, flips its target in half the tiles
+                bool affine = cx1 && ((Q >> op.q0) & 1) && !(op.cmask & Q);
+                if (affine) {
+                    const int cb = __builtin_ctzll(op.cmask);
+                    bool known = false;
+                    for (const auto &term : a_out.aff) known = known || term.first == cb;
+                    affine = known || int(a_out.aff.size()) < fused::kMaxAffine;
+                }
+                if ((perm || affine) && !fits_want) {
                     // compose A <- A o pi^-1 (pi is its own inverse)
-                    if (op.kind == G_CX) {
+                    if (affine) {
+                        const int cb = __builtin_ctzll(op.cmask);
+                        const uint32_t u = a_out.lin[local(op.q0)];
+                        bool merged = false;
+                        for (auto &term : a_out.aff)
+                            if (term.first == cb) {
+                                term.second ^= u;
+                                merged = true;
+                            }
+                        if (!merged) a_out.aff.push_back({cb, u});
+                    } else if (op.kind == G_CX) {
                         const int lc = local(__builtin_ctzll(op.cmask)), lt = local(op.q0);
-                        a_out[lc] ^= a_out[lt];
+                        a_out.lin[lc] ^= a_out.lin[lt];
                     } else {
-                        std::swap(a_out[local(op.q0)], a_out[local(op.q1)]);
+                        std::swap(a_out.lin[local(op.q0)], a_out.lin[local(op.q1)]);
                     }
                     out_pending = true;
-                    sblocked |= tq;
+                    // later ops on its qubits wait for the next stage (no op of
+                    // the pass targets an outside control)
+                    sblocked |= affine ? (uint64_t(1) << op.q0) : tq;
                     continue;
                 }
                 uint32_t need = wantm;

@@ -21,7 +21,8 @@ PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"),
                        ("q", "<i4", (16,)), ("comp", "<i4", (40,))])
 STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("relabel", "<i4"),
                         ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,)),
-                        ("rimg", "<u4", (12,)), ("rroff", "<u4", (16,))])
+                        ("rimg", "<u4", (12,)), ("rroff", "<u4", (16,)), ("naff", "<i4"), ("abit", "<u4", (8,)),
+                        ("aoff", "<u4", (8,)), ("pad2", "<i4")])
 OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4"),
                      ("pad", "<u4", (2,)), ("m", "<c16", (4,))])
 
@@ -109,10 +110,19 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
             rslots = _read_slots(st, tid, t, lb)
             assert rslots.min() >= 0 and rslots.max() < (1 << t)
             assert len(np.unique(rslots)) == (1 << t), "relabelled read must be a permutation of the tile"
+            if not int(st["relabel"]):
+                # the kernel writes a stage's slots without a barrier after its
+                # reads only when each thread writes exactly the slots it read
+                assert all(set(rslots[w].tolist()) == set(slots[w].tolist()) for w in range(W))
             if s == 0:
                 assert not int(st["relabel"]), "the first stage reads HBM, not a relabelled exchange"
             if not int(st["relabel"]):
-                assert (st["rroff"] == st["roff"]).all()
+                assert (st["rroff"] == st["roff"]).all() and int(st["naff"]) == 0
+            assert 0 <= int(st["naff"]) <= 8
+            for k in range(int(st["naff"])):
+                b = int(st["abit"][k])
+                assert b < n and b not in set(int(x) for x in P["q"][:t0]), "affine terms test tile-index bits"
+                assert int(st["aoff"][k]) < (1 << t0)
             regq = set()
             for j in range(4):
                 gj = int(st["goff"][1 << j])
@@ -210,13 +220,17 @@ def tile_base(T, q):
     return b
 
 
-def _read_slots(st, tid, t, lb):
-    """LDS slots a stage reads: its own mapping, or through the relabelling tables."""
+def _read_slots(st, tid, t, lb, base=0):
+    """LDS slots a stage reads: its own mapping, or through the relabelling
+    tables (linear images of the thread bits, affine terms on tile-index bits)."""
     if not int(st["relabel"]):
         return swz(lb)[:, None] ^ st["rroff"][None, :].astype(np.int64)
     rb = np.zeros(len(tid), dtype=np.int64)
     for i in range(t - 4):
         rb ^= ((tid >> i) & 1) * int(st["rimg"][i])
+    for k in range(int(st["naff"])):
+        if (base >> int(st["abit"][k])) & 1:
+            rb ^= int(st["aoff"][k])
     return rb[:, None] ^ st["rroff"][None, :].astype(np.int64)
 
 
@@ -256,7 +270,7 @@ def emulate(psi, n, passes, stages, ops, coef):
                     tile_idx = set(gidx.ravel().tolist())
                     v = psi[gidx]
                 else:
-                    v = lds[_read_slots(st, tid, t, lb)]
+                    v = lds[_read_slots(st, tid, t, lb, base)]
                     assert not np.isnan(v).any(), "a stage read an LDS slot nobody wrote"
                 for o in ops[st["op_begin"]: st["op_end"]]:
                     _apply(o, v, gfix, coef)

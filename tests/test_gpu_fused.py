@@ -140,3 +140,21 @@ def test_fused_config2_reduced_matches_oracle():
     want, bits = oracle.dm_run(c.ops(), 6, ref, 7, num_classical=6)
     assert ex.classical() == bits
     assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_fused_relabelled_exchanges_are_deterministic(precision):
+    # tile 12 = four waves per workgroup.  CNOTs between tile qubits, and
+    # CNOTs controlled from outside the tile, become relabellings of the LDS
+    # exchange whose reads take other threads' slots: a missing barrier would
+    # show as run-to-run differences and oracle mismatches here.
+    n = 18
+    c = ev.workloads.config1(n, 30)
+    runs = [ev.StateVectorBackend.run(c, precision=precision, tile_qubits=12).state().amplitudes() for _ in range(4)]
+    for r in runs[1:]:
+        assert np.array_equal(r, runs[0])
+    want, _ = oracle.sv_run(c.ops(), n)
+    if precision == "c128":
+        assert np.max(np.abs(runs[0] - want)) < TOL128
+    else:
+        assert np.max(np.abs(runs[0] - want)) < 1e-5
