@@ -465,11 +465,10 @@ template <class R> constexpr int kWavesPerSimd = 3;
 template <class R>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_pass(
     Amp<R> *__restrict__ psi, const Pass *__restrict__ pass, const Stage *__restrict__ stages,
-    const DevOp *__restrict__ ops, const Amp<R> *__restrict__ coef, uint64_t tiles, uint32_t xcds)
+    const DevOp *__restrict__ ops, const Amp<R> *__restrict__ coef, uint32_t per, uint32_t xcds)
 {
     extern __shared__ unsigned char lds_raw[];
-    const uint64_t per = tiles / xcds;
-    const uint64_t T = (blockIdx.x % xcds) * per + blockIdx.x / xcds;
+    const uint64_t T = uint64_t(blockIdx.x % xcds) * per + blockIdx.x / xcds;
     one_tile<R>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T),
                 reinterpret_cast<Amp<R> *>(lds_raw));
 }
@@ -503,7 +502,7 @@ void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stage
     const size_t lds = (size_t(1) << t) * sizeof(Amp<R>);
     const uint32_t xcds = (tiles % 8 == 0) ? 8u : 1u;
     hipLaunchKernelGGL(k_fused_pass<R>, dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
-                       stages, ops, static_cast<const Amp<R> *>(coef), tiles, xcds);
+                       stages, ops, static_cast<const Amp<R> *>(coef), uint32_t(tiles / xcds), xcds);
 }
 
 template <class R>

@@ -1,10 +1,11 @@
 // Autotuner for the fused pass parameters (ev_autotune in include/everett.h).
 //
 // For one state size it times complete fused runs of a synthetic brick-wall
-// circuit (HIP events around the run on the state's stream) and does
-// coordinate descent: gate budget per pass, then tile width, then the number
-// of contiguous low qubits per tile, then the budget again.  The winner is
-// recorded per (n, precision) and picked up by fused_config().
+// circuit followed by a QFT (HIP events around the run on the state's
+// stream) over a grid: the three widest tile widths, 2..5 contiguous low
+// qubits per tile and three gate budgets per pass, then the budget around
+// the winner.  The winner is recorded per (n, precision) and picked up by
+// fused_config().
 #include <hip/hip_runtime_api.h>
 
 #include <map>
@@ -44,7 +45,9 @@ ev_tune_entry autotune(int n, bool c64, int depth)
 {
     EV_REQUIRE(n >= fused::kMinTile && n <= 32, EV_E_LIMIT, "autotune needs 8..32 qubits");
     Rng rng = Rng::seed_from_u64(0x70E0ull + uint64_t(n));
-    const Circuit c = gen_brickwall(uint64_t(n), uint64_t(depth > 0 ? depth : 8), rng);
+    // the random-circuit family the benchmarks use: brick-wall layers, then a QFT
+    Circuit c = gen_brickwall(uint64_t(n), uint64_t(depth > 0 ? depth : 8), rng);
+    c.compose(gen_qft(uint64_t(n)));
     ev_sv *s = sv_create(n, c64 ? kern::C64 : kern::C128);
     ev_tune_entry best{};
     try {
@@ -72,12 +75,13 @@ ev_tune_entry autotune(int n, bool c64, int depth)
             const double ms = cost(t, low, b);
             if (ms < best.ms_per_gate) best = ev_tune_entry{t, low, b, 0, ms};
         };
-        for (uint32_t b : {6u, 10u, 14u, 28u, 40u}) try_cfg(best.tile_qubits, best.low_qubits, b);
-        for (uint32_t t = fused::kMinTile; t < tmax; ++t) try_cfg(t, best.low_qubits, best.max_gates_per_pass);
-        for (uint32_t low : {2u, 3u, 4u, 5u, 6u})
-            if (low != best.low_qubits) try_cfg(best.tile_qubits, low, best.max_gates_per_pass);
-        const uint32_t b0 = best.max_gates_per_pass;
-        for (uint32_t b : {b0 > 4 ? b0 - 4 : 2u, b0 + 4, b0 + 8}) try_cfg(best.tile_qubits, best.low_qubits, b);
+        // grid over the widest tiles, contiguous low qubits and gate budgets
+        const uint32_t t0 = std::max<uint32_t>(uint32_t(fused::kMinTile), tmax >= 2 ? tmax - 2 : tmax);
+        for (uint32_t t = t0; t <= tmax; ++t)
+            for (uint32_t low : {2u, 3u, 4u, 5u})
+                for (uint32_t b : {24u, 48u, 1000u}) try_cfg(t, low, b);
+        const uint32_t b0 = best.max_gates_per_pass;  // then the budget around the winner
+        for (uint32_t b : {b0 / 2, b0 + b0 / 2, b0 * 2}) try_cfg(best.tile_qubits, best.low_qubits, std::max(b, 2u));
     } catch (...) {
         sv_destroy(s);
         throw;

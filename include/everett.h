@@ -310,9 +310,9 @@ typedef struct {
     double ms_per_gate;
 } ev_tune_entry;
 /* Times the fused path on a synthetic brick-wall circuit of n qubits (depth
- * `depth`, 0 = 8) over candidate budgets, tile widths and low-qubit counts
- * (coordinate descent) and records the fastest for (n, precision): runs that
- * leave tile_qubits, low_qubits and max_gates_per_pass at 0 use it. */
+ * `depth`, 0 = 8) followed by QFT(n) over a grid of tile widths, low-qubit
+ * counts and gate budgets, and records the fastest for (n, precision): runs
+ * that leave tile_qubits, low_qubits and max_gates_per_pass at 0 use it. */
 int ev_autotune(uint64_t n, ev_precision p, uint32_t depth, ev_tune_entry *out);
 /* Reads or sets the recorded entry (e.g. restored from a file). */
 int ev_tune_get(uint64_t n, ev_precision p, ev_tune_entry *out, int *found);
