@@ -35,6 +35,7 @@ from .errors import (
     InvalidArgument,
     LimitError,
     OutOfMemory,
+    QasmError,
     QubitOutOfRange,
     StateError,
 )
@@ -46,7 +47,8 @@ from .state import State, measure_qubit
 __all__ = [
     "algorithms", "kernel", "workloads", "Batch", "Server", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
     "StateVectorBackend", "Execution", "DensityMatrixBackend", "DensityMatrix", "DensityMatrixExecution",
-    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "copy_bandwidth", "EverettError", "QubitOutOfRange",
+    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "copy_bandwidth", "EverettError", "QasmError",
+    "QubitOutOfRange",
     "ClassicalBitOutOfRange", "DuplicateQubit", "DimensionMismatch", "InvalidArgument", "DeviceError",
     "OutOfMemory", "AdmissionError", "LimitError", "StateError",
 ]

@@ -44,7 +44,8 @@ EV_C128, EV_C64 = 0, 1
     EV_E_ADMISSION,
     EV_E_LIMIT,
     EV_E_STATE,
-) = (0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10)
+    EV_E_QASM,
+) = (0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11)
 
 
 class EvError(C.Structure):
@@ -149,6 +150,8 @@ PROTOTYPES = {
     "ev_circuit_num_classical": (U64, [P]),
     "ev_circuit_export": (I, [P, P, SZ, C.POINTER(SZ)]),
     "ev_circuit_validate": (I, [P, C.POINTER(EvError)]),
+    "ev_circuit_from_qasm": (I, [C.c_char_p, SZ, PP, C.POINTER(EvError)]),
+    "ev_circuit_to_qasm": (I, [P, C.c_char_p, SZ, C.POINTER(SZ), C.POINTER(EvError)]),
     "ev_rng_seed": (None, [C.POINTER(EvRng), U64]),
     "ev_rng_next_u64": (U64, [C.POINTER(EvRng)]),
     "ev_rng_next_f64": (D, [C.POINTER(EvRng)]),
@@ -211,6 +214,7 @@ PROTOTYPES = {
     "ev_server_create": (I, [C.POINTER(EvServerConfig), PP]),
     "ev_server_destroy": (None, [P]),
     "ev_server_submit": (I, [P, P, U64, U64, C.POINTER(U64)]),
+    "ev_server_submit_qasm": (I, [P, C.c_char_p, SZ, U64, U64, C.POINTER(U64), C.POINTER(EvError)]),
     "ev_server_poll": (I, [P, U64, C.POINTER(I)]),
     "ev_server_wait": (I, [P, U64, P, C.POINTER(EvRequestInfo)]),
     "ev_server_get_stats": (I, [P, C.POINTER(EvServerStats)]),

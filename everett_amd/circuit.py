@@ -157,6 +157,34 @@ class Circuit:
     def z_if(self, bit, q):
         return self.if_bit(bit).z(q)
 
+    def num_ops(self) -> int:
+        """Number of ops as the reference counts them (Circuit::len, src/circuit.rs:68-71):
+        an IfClassic and the op it guards are one op."""
+        return int(np.count_nonzero(self.ops()["kind"] != _ffi.OP_IF))
+
+    # --- OpenQASM 3 -------------------------------------------------------
+    @staticmethod
+    def from_qasm(src) -> "Circuit":
+        """Circuit::from_qasm (src/circuit.rs:324-326): the reference's OpenQASM 3.0
+        subset; raises QasmError(line, col, message) on a parse error."""
+        data = src.encode("utf-8") if isinstance(src, str) else bytes(src)
+        h = C.c_void_p()
+        err = _ffi.EvError()
+        raise_for(_ffi.lib().ev_circuit_from_qasm(data, len(data), C.byref(h), C.byref(err)), err)
+        return Circuit._wrap(h)
+
+    def to_qasm(self) -> str:
+        """Circuit::to_qasm (src/circuit.rs:315-317); raises QasmError for a gate
+        with no OpenQASM 3 name or a nested IfClassic."""
+        n = C.c_size_t()
+        err = _ffi.EvError()
+        rc = _ffi.lib().ev_circuit_to_qasm(self._h, None, 0, C.byref(n), C.byref(err))
+        if rc != _ffi.EV_E_LIMIT:
+            raise_for(rc, err)
+        buf = C.create_string_buffer(n.value + 1)
+        raise_for(_ffi.lib().ev_circuit_to_qasm(self._h, buf, n.value + 1, C.byref(n), C.byref(err)), err)
+        return buf.raw[: n.value].decode("utf-8")
+
     # --- validation ------------------------------------------------------
     def validate(self) -> None:
         err = _ffi.EvError()

@@ -14,6 +14,7 @@
 #include "engine.hpp"
 #include "fused.hpp"
 #include "planner.hpp"
+#include "qasm.hpp"
 #include "server.hpp"
 #include "tune.hpp"
 
@@ -161,6 +162,38 @@ int ev_circuit_validate(const ev_circuit *c, ev_error *err)
         err);
 }
 
+static int make_circuit(ev_circuit **out, ev::Circuit &&c)
+{
+    *out = new ev_circuit{std::move(c)};
+    return EV_OK;
+}
+
+int ev_circuit_from_qasm(const char *src, size_t len, ev_circuit **out, ev_error *err)
+{
+    return guard(
+        [&] {
+            NONNULL(out);
+            EV_REQUIRE(src != nullptr || len == 0, EV_E_ARG, "src is null");
+            make_circuit(out, ev::parse_qasm(src ? src : "", len));
+        },
+        err);
+}
+
+int ev_circuit_to_qasm(const ev_circuit *c, char *buf, size_t cap, size_t *len, ev_error *err)
+{
+    return guard(
+        [&] {
+            NONNULL(c);
+            NONNULL(len);
+            const std::string text = ev::emit_qasm(c->c);
+            *len = text.size();
+            EV_REQUIRE(buf != nullptr && cap > text.size(), EV_E_LIMIT,
+                       "buffer of " + std::to_string(cap) + " bytes cannot hold " + std::to_string(text.size() + 1));
+            std::memcpy(buf, text.c_str(), text.size() + 1);
+        },
+        err);
+}
+
 void ev_rng_seed(ev_rng *r, uint64_t seed)
 {
     const ev::Rng x = ev::Rng::seed_from_u64(seed);
@@ -183,12 +216,6 @@ double ev_rng_next_f64(ev_rng *r)
     const double v = x.next_f64();
     std::memcpy(r->s, x.s, sizeof x.s);
     return v;
-}
-
-static int make_circuit(ev_circuit **out, ev::Circuit &&c)
-{
-    *out = new ev_circuit{std::move(c)};
-    return EV_OK;
 }
 
 int ev_circuit_qft(uint64_t n, ev_circuit **out)
@@ -799,6 +826,20 @@ int ev_server_submit(ev_server *s, const ev_circuit *c, uint64_t shots, uint64_t
         NONNULL(request_id);
         *request_id = ev::server_submit(s, c->c, shots, seed);
     });
+}
+
+int ev_server_submit_qasm(ev_server *s, const char *src, size_t len, uint64_t shots, uint64_t seed,
+                          uint64_t *request_id, ev_error *err)
+{
+    return guard(
+        [&] {
+            NONNULL(s);
+            NONNULL(request_id);
+            EV_REQUIRE(src != nullptr || len == 0, EV_E_ARG, "src is null");
+            const ev::Circuit c = ev::parse_qasm(src ? src : "", len);
+            *request_id = ev::server_submit(s, c, shots, seed);
+        },
+        err);
 }
 
 int ev_server_poll(ev_server *s, uint64_t request_id, int *done)

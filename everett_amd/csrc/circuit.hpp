@@ -77,6 +77,13 @@ class Circuit {
     void append(const ev_op &op);
     void compose(const Circuit &sub);  // circuit.rs:124-127
     void to_ev(size_t i, ev_op &out) const;
+    // Register sizes set after the ops (the OpenQASM reader learns them from
+    // declarations that may follow the statements).
+    void set_registers(uint64_t num_qubits, uint64_t num_classical)
+    {
+        nq_ = num_qubits;
+        ncl_ = num_classical;
+    }
 
     // circuit.rs:245-306; throws Error with the reference's first failure.
     void validate() const;

@@ -64,6 +64,17 @@ class StateError(EverettError):
     """The handle is not in a state that allows the call."""
 
 
+class QasmError(EverettError, ValueError):
+    """Error::Qasm (src/error.rs:44-52): an OpenQASM 3 parse or emit error at
+    line:col (1-based; 0 where none applies)."""
+
+    def __init__(self, line: int, col: int, msg: str = ""):
+        super().__init__(msg or f"OpenQASM 3 error at {line}:{col}")
+        self.line, self.col = line, col
+        prefix = f"OpenQASM 3 error at {line}:{col}: "
+        self.message = msg[len(prefix):] if msg.startswith(prefix) else msg
+
+
 def raise_for(rc: int, err: "_ffi.EvError | None" = None) -> None:
     """Raises the exception matching a C ABI status code (no-op for EV_OK)."""
     if rc == _ffi.EV_OK:
@@ -89,6 +100,8 @@ def raise_for(rc: int, err: "_ffi.EvError | None" = None) -> None:
         raise LimitError(msg)
     if rc == _ffi.EV_E_STATE:
         raise StateError(msg)
+    if rc == _ffi.EV_E_QASM:
+        raise QasmError(a, b, msg)
     raise InvalidArgument(msg)
 
 

@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from .errors import check
+from .errors import check, raise_for
 from .state import precision_code
 
 
@@ -46,6 +46,16 @@ class Server:
     def submit(self, circuit, shots: int = 1024, seed: int = 0) -> int:
         rid = C.c_uint64()
         check(_ffi.lib().ev_server_submit(self._h, circuit.handle, shots, seed & 0xFFFFFFFFFFFFFFFF, C.byref(rid)))
+        self._shots[rid.value] = shots
+        return rid.value
+
+    def submit_qasm(self, src, shots: int = 1024, seed: int = 0) -> int:
+        """Submits a request given as OpenQASM 3 text (the serving wire format)."""
+        data = src.encode("utf-8") if isinstance(src, str) else bytes(src)
+        rid = C.c_uint64()
+        err = _ffi.EvError()
+        raise_for(_ffi.lib().ev_server_submit_qasm(self._h, data, len(data), shots, seed & 0xFFFFFFFFFFFFFFFF,
+                                                     C.byref(rid), C.byref(err)), err)
         self._shots[rid.value] = shots
         return rid.value
 

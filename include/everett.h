@@ -48,7 +48,8 @@ typedef enum {
     EV_E_HIP = -7,         /* HIP runtime error */
     EV_E_ADMISSION = -8,   /* serving: the request can never fit the configured memory */
     EV_E_LIMIT = -9,       /* beyond a documented size limit of this build */
-    EV_E_STATE = -10       /* the handle is not in a state that allows the call */
+    EV_E_STATE = -10,      /* the handle is not in a state that allows the call */
+    EV_E_QASM = -11        /* Error::Qasm                   src/error.rs:44-52  a=line  b=col (1-based; 0 = n/a) */
 } ev_status;
 
 /* Structured error (mirrors src/error.rs:8-53). */
@@ -99,6 +100,19 @@ uint64_t ev_circuit_num_qubits(const ev_circuit *c);                   /* circui
 uint64_t ev_circuit_num_classical(const ev_circuit *c);                /* circuit.rs:62-65 */
 int ev_circuit_export(const ev_circuit *c, ev_op *out, size_t cap, size_t *len);
 int ev_circuit_validate(const ev_circuit *c, ev_error *err);           /* circuit.rs:245-306 */
+
+/* OpenQASM 3.0 text (the reference's subset: stdgates h x y z s t id rx ry
+ * rz p, cx cz swap, ctrl(k) @, both measure forms, if (c[i] == 1)).
+ * ev_circuit_from_qasm replaces Circuit::from_qasm (src/circuit.rs:324-326,
+ * src/qasm/parse.rs:811-819): on a parse error it returns EV_E_QASM with
+ * err->a = line, err->b = col and err->message = "OpenQASM 3 error at
+ * <line>:<col>: <message>" (src/error.rs:82-84).  ev_circuit_to_qasm replaces
+ * Circuit::to_qasm (src/circuit.rs:315-317, src/qasm/emit.rs:17-37): it
+ * writes the text and a NUL into buf and its length (without the NUL) into
+ * *len; EV_E_LIMIT (with *len set) when cap is too small, EV_E_QASM for a
+ * gate with no OpenQASM 3 name or a nested IfClassic. */
+int ev_circuit_from_qasm(const char *src, size_t len, ev_circuit **out, ev_error *err);
+int ev_circuit_to_qasm(const ev_circuit *c, char *buf, size_t cap, size_t *len, ev_error *err);
 
 /* src/rng.rs:9-56 (SplitMix64-seeded xoshiro256++), host only. */
 typedef struct { uint64_t s[4]; } ev_rng;
@@ -294,6 +308,12 @@ int ev_server_create(const ev_server_config *cfg, ev_server **out);
 void ev_server_destroy(ev_server *s);   /* drains in-flight work, then frees */
 /* Copies the circuit; returns EV_E_ADMISSION if it can never fit the budget. */
 int ev_server_submit(ev_server *s, const ev_circuit *c, uint64_t shots, uint64_t seed, uint64_t *request_id);
+/* Serving wire format: parses OpenQASM 3 text as ev_circuit_from_qasm does
+ * and submits the circuit as ev_server_submit does; a parse error returns
+ * EV_E_QASM before anything is queued.  (The reference has no server; its
+ * text entry point is Circuit::from_qasm, src/circuit.rs:324-326.) */
+int ev_server_submit_qasm(ev_server *s, const char *src, size_t len, uint64_t shots, uint64_t seed,
+                          uint64_t *request_id, ev_error *err);
 /* *done = 1 once the request has completed (or failed). */
 int ev_server_poll(ev_server *s, uint64_t request_id, int *done);
 /* Blocks until the request completes; copies its samples (shots entries) and

@@ -96,6 +96,10 @@ def lib():
             "ref_gen_dm_random": (SZ, [SZ, SZ, R, P]),
             "ref_sample": (None, [P, SZ, P, SZ, P]),
         }
+        for g in ("h", "x", "y", "z", "s", "t", "cnot", "cz", "swap"):
+            protos[f"ref_gate_{g}"] = (None, [P])
+        for g in ("rx", "ry", "rz", "phase"):
+            protos[f"ref_gate_{g}"] = (None, [D, P])
         for name, (res, args) in protos.items():
             fn = getattr(h, name)
             fn.restype = res
@@ -106,6 +110,27 @@ def lib():
 
 def _p(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
+
+
+def gate1(name: str, angle: float | None = None) -> np.ndarray:
+    """src/gate.rs constructors (restated in refcpu.c): 2x2 row-major as 4 complex."""
+    m = np.zeros(4, dtype=np.complex128)
+    if name == "id":
+        m[0] = m[3] = 1.0  # gate.rs Gate1::id
+        return m
+    fn = getattr(lib(), f"ref_gate_{name}")
+    if angle is None:
+        fn(_p(m))
+    else:
+        fn(float(angle), _p(m))
+    return m
+
+
+def gate2(name: str) -> np.ndarray:
+    """Gate2::cnot / cz / swap (src/gate.rs:190-232) as 16 complex, basis |q_a q_b>."""
+    m = np.zeros(16, dtype=np.complex128)
+    getattr(lib(), f"ref_gate_{name}")(_p(m))
+    return m
 
 
 def set_threads(n: int) -> None:

@@ -87,3 +87,28 @@ def test_empty_circuit_request_samples_zero_state():
         rid = srv.submit(ev.Circuit(9), shots=8, seed=1)
         samples, info = srv.wait(rid)
     assert info["status"] == 0 and list(samples) == [0] * 8
+
+
+def test_requests_in_openqasm_wire_format():
+    # requests arrive as OpenQASM 3 text and are parsed on submit: a text request
+    # samples exactly like its parsed circuit submitted directly, a text that does
+    # not parse is refused before anything is queued, and the text reproduces the
+    # original circuit's state to 1e-10 fidelity (angles travel as 17-decimal text)
+    reqs = requests(12, 8, 13, seed=4242)
+    texts = [c.to_qasm() for c, _ in reqs]
+    with ev.Server(precision="c64") as srv:
+        ids = [srv.submit_qasm(t, shots=256, seed=s) for t, (c, s) in zip(texts, reqs)]
+        ids2 = [srv.submit(ev.Circuit.from_qasm(t), shots=256, seed=s) for t, (c, s) in zip(texts, reqs)]
+        a = [srv.wait(i)[0] for i in ids]
+        b = [srv.wait(i)[0] for i in ids2]
+        with pytest.raises(ev.QasmError) as e:
+            srv.submit_qasm("qubit[2] q;\nfoo q[0];\n")
+        assert (e.value.line, e.value.col) == (3, 1)
+        st = srv.stats()
+    assert st["submitted"] == 2 * len(reqs) and st["completed"] == 2 * len(reqs) and st["failed"] == 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    for (c, _), t in zip(reqs, texts):
+        s1 = ev.StateVectorBackend.run(c).state()
+        s2 = ev.StateVectorBackend.run(ev.Circuit.from_qasm(t)).state()
+        assert s1.fidelity(s2) > 1 - 1e-10
