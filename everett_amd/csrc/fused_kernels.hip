@@ -24,34 +24,36 @@ namespace {
 
 template <class V> using Regs = V[kRegs];
 
-// Op bodies.  Each output component is one fused multiply-add chain (one
-// rounding per step) whose last step reads the input component it replaces.
+// Op bodies.  Every op reads the thread's 16 amplitudes from `in` and
+// writes all 16 to `out` (the op loop alternates two register arrays).
+// Writing out of place lets every op kind's code leave its results in the
+// same registers, so the loop carries both arrays without copies; in-place
+// updates made the register allocator give each op kind's results other
+// registers than the loop head's and copy the whole array back after every
+// op.  Each output component is one fused multiply-add chain (one rounding
+// per step) whose last step adds the replaced input's own term.
 
-// (a0, a1) <- (m0 a0 + m1 a1, m2 a0 + m3 a1).  Per output component one
-// chain: a0.re = m0.re a0.re + (-m0.im a0.im + (m1.re a1.re + -(m1.im a1.im)))
-// and likewise, the term of the replaced input last.
-template <bool MS, class V> __device__ __forceinline__ void mix2(V &a0, V &a1, V m0, V m1, V m2, V m3)
+// (o0, o1) <- (m0 a0 + m1 a1, m2 a0 + m3 a1).  Per output component one
+// chain: o0.re = m0.re a0.re + (-m0.im a0.im + (m1.re a1.re + -(m1.im a1.im)))
+// and likewise.
+template <class V>
+__device__ __forceinline__ void mix2(const V &a0, const V &a1, V &o0, V &o1, V m0, V m1, V m2, V m3)
 {
     const auto p0x = dfma(-m0.y, a0.y, dfma(m1.x, a1.x, -(m1.y * a1.y)));
     const auto p0y = dfma(m0.y, a0.x, dfma(m1.x, a1.y, m1.y * a1.x));
     const auto p1x = dfma(m2.x, a0.x, dfma(-m2.y, a0.y, -(m3.y * a1.y)));
     const auto p1y = dfma(m2.x, a0.y, dfma(m2.y, a0.x, m3.y * a1.x));
-    a0 = mk<V>(dfma(m0.x, a0.x, p0x), dfma(m0.x, a0.y, p0y));
-    a1 = mk<V>(dfma(m3.x, a1.x, p1x), dfma(m3.x, a1.y, p1y));
+    o0 = mk<V>(dfma(m0.x, a0.x, p0x), dfma(m0.x, a0.y, p0y));
+    o1 = mk<V>(dfma(m3.x, a1.x, p1x), dfma(m3.x, a1.y, p1y));
 }
 
-// a <- d a with the reference's complex product (src/complex.rs:115-125):
-// (fma(d.re, a.re, -(d.im a.im)), fma(d.re, a.im, d.im a.re)).
-template <bool MS, class V> __device__ __forceinline__ void scale_by(V &a, V d)
-{
-    a = cmul(d, a);
-}
-
-// a_r <- sum_c m[4r + c] a_c for a group of four registers.
-template <class V> __device__ __forceinline__ void mix4(V &a0, V &a1, V &a2, V &a3, const V *m)
+// o_r <- sum_c m[4r + c] a_c for a group of four registers.
+template <class V>
+__device__ __forceinline__ void mix4(const V &a0, const V &a1, const V &a2, const V &a3, V &o0, V &o1, V &o2, V &o3,
+                                     const V *m)
 {
     const V x[4] = {a0, a1, a2, a3};
-    decltype(V::x) px[4], py[4];
+    V y[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const V *row = m + 4 * r;
@@ -62,13 +64,12 @@ template <class V> __device__ __forceinline__ void mix4(V &a0, V &a1, V &a2, V &
             ax = dfma(row[c].x, x[c].x, dfma(-row[c].y, x[c].y, ax));
             ay = dfma(row[c].x, x[c].y, dfma(row[c].y, x[c].x, ay));
         }
-        px[r] = ax;
-        py[r] = ay;
+        y[r] = mk<V>(dfma(row[r].x, x[r].x, ax), dfma(row[r].x, x[r].y, ay));
     }
-    a0 = mk<V>(dfma(m[0].x, x[0].x, px[0]), dfma(m[0].x, x[0].y, py[0]));
-    a1 = mk<V>(dfma(m[5].x, x[1].x, px[1]), dfma(m[5].x, x[1].y, py[1]));
-    a2 = mk<V>(dfma(m[10].x, x[2].x, px[2]), dfma(m[10].x, x[2].y, py[2]));
-    a3 = mk<V>(dfma(m[15].x, x[3].x, px[3]), dfma(m[15].x, x[3].y, py[3]));
+    o0 = y[0];
+    o1 = y[1];
+    o2 = y[2];
+    o3 = y[3];
 }
 
 // Workgroup barrier for the LDS exchanges between stages.  It orders LDS
@@ -94,82 +95,110 @@ template <class V> __device__ __forceinline__ V pick(bool c, V a, V b)
 template <class V> __device__ __forceinline__ V one_unless(bool tok, V m) { return pick(tok, m, mk<V>(1, 0)); }
 template <class V> __device__ __forceinline__ V zero_unless(bool tok, V m) { return pick(tok, m, mk<V>(0, 0)); }
 
-template <bool MS, class V, int B>
-__device__ __forceinline__ void op_u1(Regs<V> &v, V m0, V m1, V m2, V m3, uint32_t creg)
+// Register conditions: (j & creg) == creg, decided per register by a
+// uniform branch; registers an op leaves alone are copied.
+template <class V, int B>
+__device__ __forceinline__ void op_u1(const Regs<V> &in, Regs<V> &out, V m0, V m1, V m2, V m3, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & (1 << B)) continue;
-        if ((uint32_t(j) & creg) != creg) continue;
-        mix2<MS>(v[j], v[j | (1 << B)], m0, m1, m2, m3);
-    }
-}
-
-template <class V, int B> __device__ __forceinline__ void op_cx(Regs<V> &v, uint32_t creg, bool tok)
-{
-#pragma unroll
-    for (int j = 0; j < kRegs; ++j) {
-        if (j & (1 << B)) continue;
-        if ((uint32_t(j) & creg) != creg) continue;
         const int j1 = j | (1 << B);
-        const V a0 = v[j], a1 = v[j1];
-        v[j] = pick(tok, a1, a0);
-        v[j1] = pick(tok, a0, a1);
+        if ((uint32_t(j) & creg) == creg) {
+            mix2(in[j], in[j1], out[j], out[j1], m0, m1, m2, m3);
+        } else {
+            out[j] = in[j];
+            out[j1] = in[j1];
         }
+    }
 }
 
-template <class V, int B> __device__ __forceinline__ void op_x(Regs<V> &v, uint32_t creg)
+// A register value the compiler cannot trace back to its array element (an
+// empty asm, no instruction): a select between two such values stays a
+// select of values.  Selecting between the array elements themselves is
+// turned into a load through a selected address, which moves both register
+// arrays to scratch memory.
+template <class V> __device__ __forceinline__ V opaque(V a)
+{
+    asm("" : "+v"(a.x), "+v"(a.y));
+    return a;
+}
+
+// Multi-controlled X: exchange the pair where the register condition holds
+// and the lane's global condition tok holds.
+template <class V, int B> __device__ __forceinline__ void op_cx(const Regs<V> &in, Regs<V> &out, uint32_t creg, bool tok)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & (1 << B)) continue;
-        if ((uint32_t(j) & creg) != creg) continue;
         const int j1 = j | (1 << B);
-        const V t = v[j];
-        v[j] = v[j1];
-        v[j1] = t;
+        if ((uint32_t(j) & creg) == creg) {
+            const V a = opaque(in[j]), b = opaque(in[j1]);
+            out[j] = pick(tok, b, a);
+            out[j1] = pick(tok, a, b);
+        } else {
+            out[j] = in[j];
+            out[j1] = in[j1];
+        }
     }
 }
 
-template <bool MS, class V, int B>
-__device__ __forceinline__ void op_diagr(Regs<V> &v, V d0, V d1, bool d0_one, uint32_t creg)
+template <class V, int B> __device__ __forceinline__ void op_x(const Regs<V> &in, Regs<V> &out, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
-        if ((uint32_t(j) & creg) != creg) continue;
-        if (j & (1 << B)) scale_by<MS>(v[j], d1);
-        else if (!d0_one) scale_by<MS>(v[j], d0);
+        if (j & (1 << B)) continue;
+        const int j1 = j | (1 << B);
+        const bool c = (uint32_t(j) & creg) == creg;
+        const V a = opaque(in[j]), b = opaque(in[j1]);
+        out[j] = pick(c, b, a);
+        out[j1] = pick(c, a, b);
     }
 }
 
-template <class V> __device__ __forceinline__ void op_diagt(Regs<V> &v, V d, uint32_t creg)
+// a <- d a with the reference's complex product (src/complex.rs:115-125).
+template <class V, int B>
+__device__ __forceinline__ void op_diagr(const Regs<V> &in, Regs<V> &out, V d0, V d1, bool d0_one, uint32_t creg)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
-        if ((uint32_t(j) & creg) != creg) continue;
-        scale_by<false>(v[j], d);
+        if ((uint32_t(j) & creg) != creg) out[j] = in[j];
+        else if (j & (1 << B)) out[j] = cmul(d1, in[j]);
+        else if (!d0_one) out[j] = cmul(d0, in[j]);
+        else out[j] = in[j];
+    }
+}
+
+template <class V> __device__ __forceinline__ void op_diagt(const Regs<V> &in, Regs<V> &out, V d, uint32_t creg)
+{
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        if ((uint32_t(j) & creg) == creg) out[j] = cmul(d, in[j]);
+        else out[j] = in[j];
     }
 }
 
 // Dense 4x4 in the basis |b_hi b_lo> (b_hi = MSB).
-template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(Regs<V> &v, const V *m)
+template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(const Regs<V> &in, Regs<V> &out, const V *m)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & ((1 << BH) | (1 << BL))) continue;
-        mix4(v[j], v[j | (1 << BL)], v[j | (1 << BH)], v[j | (1 << BH) | (1 << BL)], m);
+        const int a = j, b = j | (1 << BL), c = j | (1 << BH), d = j | (1 << BH) | (1 << BL);
+        mix4(in[a], in[b], in[c], in[d], out[a], out[b], out[c], out[d], m);
     }
 }
 
-template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<V> &v)
+template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(const Regs<V> &in, Regs<V> &out)
 {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         if (j & ((1 << BH) | (1 << BL))) continue;
         const int a = j | (1 << BL), b = j | (1 << BH);
-        const V t = v[a];
-        v[a] = v[b];
-        v[b] = t;
+        out[j] = in[j];
+        out[a] = in[b];
+        out[b] = in[a];
+        out[a | b] = in[a | b];
     }
 }
 
@@ -179,14 +208,6 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(Regs<
     case 1: fn<V, 1>(__VA_ARGS__); break;   \
     case 2: fn<V, 2>(__VA_ARGS__); break;   \
     default: fn<V, 3>(__VA_ARGS__); break;  \
-    }
-
-#define EV_CASE_B_MS(fn, MS, ...)                \
-    switch (b0) {                                \
-    case 0: fn<MS, V, 0>(__VA_ARGS__); break;    \
-    case 1: fn<MS, V, 1>(__VA_ARGS__); break;    \
-    case 2: fn<MS, V, 2>(__VA_ARGS__); break;    \
-    default: fn<MS, V, 3>(__VA_ARGS__); break;   \
     }
 
 #define EV_CASE_PAIR(fn, ...)                        \
@@ -243,56 +264,50 @@ template <class V> __device__ __forceinline__ void load_row(cptr<V> m, V *out)
 }
 
 template <class V>
-__device__ __forceinline__ void run_op(const OpView<V> &op, Regs<V> &v, uint64_t gfix, cptr<V> coef)
+__device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, Regs<V> &out, uint64_t gfix,
+                                       cptr<V> coef)
 {
     const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
     const uint32_t flags = op.code >> 24;
     const bool tok = (gfix & op.cglob) == op.cglob;
     switch (kind) {
-    case kU1: EV_CASE_B_MS(op_u1, true, v, op.m[0], op.m[1], op.m[2], op.m[3], 0u); break;
-    case kCU1:
-        if (op.cglob == 0) {
-            EV_CASE_B_MS(op_u1, true, v, op.m[0], op.m[1], op.m[2], op.m[3], op.creg);
-        } else {
-            const V m0 = one_unless(tok, op.m[0]), m1 = zero_unless(tok, op.m[1]);
-            const V m2 = zero_unless(tok, op.m[2]), m3 = one_unless(tok, op.m[3]);
-            EV_CASE_B_MS(op_u1, false, v, m0, m1, m2, m3, op.creg);
-        }
+    case kU1: EV_CASE_B(op_u1, in, out, op.m[0], op.m[1], op.m[2], op.m[3], 0u); break;
+    case kCU1: {
+        // a lane whose global condition fails multiplies by the identity (exact)
+        const V m0 = one_unless(tok, op.m[0]), m1 = zero_unless(tok, op.m[1]);
+        const V m2 = zero_unless(tok, op.m[2]), m3 = one_unless(tok, op.m[3]);
+        EV_CASE_B(op_u1, in, out, m0, m1, m2, m3, op.creg);
         break;
+    }
     case kCX:
         if (op.cglob == 0) {
-            EV_CASE_B(op_x, v, op.creg);
+            EV_CASE_B(op_x, in, out, op.creg);
         } else {
-            EV_CASE_B(op_cx, v, op.creg, tok);
+            EV_CASE_B(op_cx, in, out, op.creg, tok);
         }
         break;
     case kDiagR: {
         const bool one = (flags & kD0IsOne) != 0;
-        if (op.cglob == 0) {
-            EV_CASE_B_MS(op_diagr, true, v, op.m[0], op.m[1], one, op.creg);
-        } else {
-            const V d0 = one_unless(tok, op.m[0]), d1 = one_unless(tok, op.m[1]);
-            EV_CASE_B_MS(op_diagr, false, v, d0, d1, one, op.creg);
-        }
+        const V d0 = one_unless(tok, op.m[0]), d1 = one_unless(tok, op.m[1]);
+        EV_CASE_B(op_diagr, in, out, d0, d1, one, op.creg);
         break;
     }
     case kDiagT: {
         const V d = one_unless(tok, pick(((gfix >> op.tq) & 1) != 0, op.m[1], op.m[0]));
-        op_diagt<V>(v, d, op.creg);
+        op_diagt<V>(in, out, d, op.creg);
         break;
     }
     case kU2: {
         V m[16];
         load_row<V>(coef + op.coef, m);
-        EV_CASE_PAIR(op_u2, v, m);
+        EV_CASE_PAIR(op_u2, in, out, m);
         break;
     }
     case kPhase: {
         // the lane's factor: product of the factors whose mask it meets
         const cptr<uint64_t> masks = (cptr<uint64_t>)(op.p->m);
-        const cptr<V> fac = coef + op.coef;
         using F = decltype(V::x);
-        const cptr<F> ff = (cptr<F>)(fac);
+        const cptr<F> ff = (cptr<F>)(coef + op.coef);
         V f = pick((gfix & masks[0]) == masks[0], mk<V>(ff[0], ff[1]), mk<V>(1, 0));
         for (uint32_t i = 1; i < b1; ++i) {
             const uint64_t mk_i = masks[i];
@@ -300,13 +315,13 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, Regs<V> &v, uint64_t
             f = pick((gfix & mk_i) == mk_i, g, f);
         }
         if (b0 == kPhaseAll) {
-            op_diagt<V>(v, f, 0u);
+            op_diagt<V>(in, out, f, 0u);
         } else {
-            EV_CASE_B_MS(op_diagr, false, v, mk<V>(1, 0), f, true, 0u);
+            EV_CASE_B(op_diagr, in, out, mk<V>(1, 0), f, true, 0u);
         }
         break;
     }
-    default: EV_CASE_PAIR(op_swap, v); break;
+    default: EV_CASE_PAIR(op_swap, in, out); break;
     }
 }
 
@@ -379,16 +394,24 @@ __device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Re
         }
 }
 
+// The stage's ops alternate between v and w, two at a time (the next op's
+// descriptor loads are in flight while the current op computes); an odd
+// last op is followed by one copy back to v.
 template <class V>
-__device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, uint64_t gfix, cptr<DevOp> ops, cptr<V> coef)
+__device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V> &w, uint64_t gfix, cptr<DevOp> ops,
+                                              cptr<V> coef)
 {
     const int ob = __builtin_amdgcn_readfirstlane(st->op_begin), oe = __builtin_amdgcn_readfirstlane(st->op_end);
-    if (ob >= oe) return;
-    OpView<V> cur = fetch_op<V>(ops + ob);
-    for (int o = ob; o < oe; ++o) {
-        const OpView<V> nxt = fetch_op<V>(ops + ((o + 1 < oe) ? o + 1 : o));
-        run_op<V>(cur, v, gfix, coef);
-        cur = nxt;
+    int o = ob;
+    for (; o + 1 < oe; o += 2) {
+        const OpView<V> a = fetch_op<V>(ops + o), b = fetch_op<V>(ops + o + 1);
+        run_op<V>(a, v, w, gfix, coef);
+        run_op<V>(b, w, v, gfix, coef);
+    }
+    if (o < oe) {
+        run_op<V>(fetch_op<V>(ops + o), v, w, gfix, coef);
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) v[j] = w[j];
     }
 }
 
@@ -404,6 +427,7 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
     using V = Amp<R>;
     const bool active = tid < (1u << (tr.t - kRegBits));
     const int nst = tr.nstages;
+    Regs<V> w;  // the op loop's second register array
     for (int s = 0; s < nst; ++s) {
         const cptr<Stage> st = tr.stages + s;
         const Lane ln = lane_of(st, tid, tr.base);
@@ -419,7 +443,7 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
 #pragma unroll
             for (int j = 0; j < kRegs; ++j) v[j] = lds[rb ^ st->rroff[j]];
         }
-        run_stage_ops<V>(st, v, ln.gfix, tr.ops, tr.coef);
+        run_stage_ops<V>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
             // A relabelled read took other threads' slots: every thread must
             // be done reading before any overwrites them.  (Otherwise each
