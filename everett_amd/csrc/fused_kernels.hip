@@ -415,6 +415,57 @@ __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V
     }
 }
 
+// complex128 exchanges move the real parts, then the imaginary parts,
+// through one LDS buffer of 2^t doubles: half the LDS a whole-amplitude
+// buffer takes, which keeps 12 waves per CU resident (the register limit)
+// instead of 10 at 10-qubit tiles and 8 at 12-qubit tiles.  complex64 tiles
+// exchange whole amplitudes (their LDS is already half).
+template <class R> constexpr bool kSplitExchange = sizeof(R) == 8;
+template <class R> constexpr size_t kLdsSlotBytes = kSplitExchange<R> ? sizeof(R) : sizeof(Amp<R>);
+
+// LDS exchange between two stages: this thread's registers leave through
+// slots ws ^ wroff[j] and the next stage's arrive from slots rs ^ rroff[j].
+template <class R>
+__device__ __forceinline__ void exchange(Regs<Amp<R>> &v, void *lds, bool active, uint32_t ws, cptr<uint32_t> wroff,
+                                         uint32_t rs, cptr<uint32_t> rroff)
+{
+    using V = Amp<R>;
+    if constexpr (kSplitExchange<R>) {
+        R *buf = static_cast<R *>(lds);
+        R re[kRegs];
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j].x;
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) re[j] = buf[rs ^ rroff[j]];
+        }
+        lds_barrier();  // every real part read before the imaginary parts overwrite them
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j].y;
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) v[j] = mk<V>(re[j], buf[rs ^ rroff[j]]);
+        }
+    } else {
+        V *buf = static_cast<V *>(lds);
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j];
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) v[j] = buf[rs ^ rroff[j]];
+        }
+    }
+}
+
 // Runs one tile whose first-stage amplitudes are already in v: stages with
 // LDS exchanges between them, then the global scatter in the last stage's
 // mapping.  Threads beyond the tile (a batch launch sizes workgroups for its
@@ -422,43 +473,36 @@ __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V
 // every memory access, and keep every barrier; the op loop thus stays out of
 // divergent control flow and its descriptor reads stay scalar.
 template <class R>
-__device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v, Amp<R> *lds)
+__device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v, void *lds)
 {
     using V = Amp<R>;
     const bool active = tid < (1u << (tr.t - kRegBits));
     const int nst = tr.nstages;
     Regs<V> w;  // the op loop's second register array
+    Lane ln = lane_of(tr.stages, tid, tr.base);
     for (int s = 0; s < nst; ++s) {
         const cptr<Stage> st = tr.stages + s;
-        const Lane ln = lane_of(st, tid, tr.base);
-        if (s > 0 && active) {
-            uint32_t rb = ln.sb;
-            if (st->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before this stage
-                rb = 0;
-                for (int i = 0; i < st->np; ++i)
-                    if ((tid >> i) & 1u) rb ^= st->rimg[i];
-                for (int k = 0; k < st->naff; ++k)  // CNOTs controlled by the tile index
-                    if ((tr.base >> st->abit[k]) & 1u) rb ^= st->aoff[k];
-            }
-#pragma unroll
-            for (int j = 0; j < kRegs; ++j) v[j] = lds[rb ^ st->rroff[j]];
-        }
         run_stage_ops<V>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
-            // A relabelled read took other threads' slots: every thread must
-            // be done reading before any overwrites them.  (Otherwise each
-            // thread writes exactly the slots it read.)
-            if (s > 0 && st->relabel) lds_barrier();
-            if (active) {
-#pragma unroll
-                for (int j = 0; j < kRegs; ++j) lds[ln.sb ^ st->roff[j]] = v[j];
+            const cptr<Stage> nx = st + 1;
+            const Lane nl = lane_of(nx, tid, tr.base);
+            uint32_t rb = nl.sb;
+            if (nx->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before the next stage
+                rb = 0;
+                for (int i = 0; i < nx->np; ++i)
+                    if ((tid >> i) & 1u) rb ^= nx->rimg[i];
+                for (int k = 0; k < nx->naff; ++k)  // CNOTs controlled by the tile index
+                    if ((tr.base >> nx->abit[k]) & 1u) rb ^= nx->aoff[k];
             }
-            lds_barrier();
+            // A relabelled read into this stage took other threads' slots:
+            // every thread must be done reading before any overwrites them.
+            // (Otherwise each thread writes exactly the slots it read.)
+            if (s > 0 && st->relabel) lds_barrier();
+            exchange<R>(v, lds, active, ln.sb, st->roff, rb, nx->rroff);
+            ln = nl;
         } else if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) {
-                tr.psi[ln.gfix | st->goff[j]] = v[j];
-                        }
+            for (int j = 0; j < kRegs; ++j) tr.psi[ln.gfix | st->goff[j]] = v[j];
         }
     }
 }
@@ -469,7 +513,7 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
 // measured 1.7x slower on MI355X (any CU slot it fails to place delays a
 // whole strip of tiles).
 template <class R>
-__device__ __forceinline__ void one_tile(const TileRef<R> &tr, Amp<R> *lds)
+__device__ __forceinline__ void one_tile(const TileRef<R> &tr, void *lds)
 {
     using V = Amp<R>;
     const uint32_t tid = threadIdx.x;
@@ -478,9 +522,10 @@ __device__ __forceinline__ void one_tile(const TileRef<R> &tr, Amp<R> *lds)
     compute_tile<R>(tr, tid, v, lds);
 }
 
-// Register budget: 168 VGPRs (3 waves per SIMD).  A c128 tile of 2^10
-// amplitudes takes 16 KiB of LDS, so LDS allows 10 single-wave workgroups
-// per CU (2.5 per SIMD); registers are not the tighter limit.
+// Register budget: 168 VGPRs (3 waves per SIMD; the two c128 register
+// arrays alone take 128).  With the split exchange a c128 tile of 2^t
+// amplitudes takes 2^t * 8 bytes of LDS, so 12 waves per CU fit at every
+// tile width.
 template <class R> constexpr int kWavesPerSimd = 3;
 
 // XCD-aware tile order: dispatch places workgroup b on XCD b mod 8, so XCD x
@@ -493,8 +538,7 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
 {
     extern __shared__ unsigned char lds_raw[];
     const uint64_t T = uint64_t(blockIdx.x % xcds) * per + blockIdx.x / xcds;
-    one_tile<R>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T),
-                reinterpret_cast<Amp<R> *>(lds_raw));
+    one_tile<R>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
 }
 
 // Batch kernel: workgroup b runs linear tile b of the round, whose entry
@@ -514,7 +558,7 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
     const cptr<BatchEntry> e = ent + lo;
     one_tile<R>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
                             as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin),
-                reinterpret_cast<Amp<R> *>(lds_raw));
+                lds_raw);
 }
 
 template <class R>
@@ -523,7 +567,7 @@ void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stage
 {
     const uint64_t tiles = uint64_t(1) << (n - t);
     const unsigned block = 1u << (t - kRegBits);  // <= kMaxThreads = the kernel's launch bound
-    const size_t lds = (size_t(1) << t) * sizeof(Amp<R>);
+    const size_t lds = (size_t(1) << t) * kLdsSlotBytes<R>;
     const uint32_t xcds = (tiles % 8 == 0) ? 8u : 1u;
     hipLaunchKernelGGL(k_fused_pass<R>, dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
                        stages, ops, static_cast<const Amp<R> *>(coef), uint32_t(tiles / xcds), xcds);
@@ -533,7 +577,7 @@ template <class R>
 void launch_batch_t(const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st)
 {
     const unsigned block = 1u << (max_t - kRegBits);
-    const size_t lds = (size_t(1) << max_t) * sizeof(Amp<R>);
+    const size_t lds = (size_t(1) << max_t) * kLdsSlotBytes<R>;
     hipLaunchKernelGGL(k_fused_batch<R>, dim3(unsigned(total_tiles)), dim3(block), lds, st, entries, nentries);
 }
 
