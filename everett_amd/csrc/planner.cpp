@@ -697,27 +697,37 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
     std::vector<int> rem(g.size());
     for (size_t i = 0; i < g.size(); ++i) rem[i] = int(i);
 
-    while (!rem.empty()) {
-        uint64_t Q = lowmask, blocked = 0;
-        float cost = 0;
+    // One pass's ops: in program order an op joins when the qubits it needs
+    // fit the tile and it does not depend on a deferred op.  `grow` lets the
+    // tile take new qubits up to t (program-order greedy); otherwise the tile
+    // is the fixed set Q0.
+    struct Formed {
+        uint64_t Q;
+        float work;
         std::vector<int> take, defer;
+    };
+    auto form = [&](uint64_t Q0, bool grow) {
+        Formed f{Q0, 0.f, {}, {}};
+        uint64_t blocked = 0;
+        float cost = 0;
         size_t i = 0;
         for (; i < rem.size(); ++i) {
             const GOp &op = g[rem[i]];
             const uint64_t tq = op.touched();
             if (tq & blocked) {
                 blocked |= tq;
-                defer.push_back(rem[i]);
+                f.defer.push_back(rem[i]);
             } else {
-                uint64_t nq = Q | op.want();
-                if (popc(nq) > t) nq = Q | op.need();
-                if (popc(nq) <= t && (take.empty() || cost + op.cost <= budget)) {
-                    Q = nq;
+                uint64_t nq = f.Q | op.want();
+                if (!grow || popc(nq) > t) nq = f.Q | op.need();
+                const bool fits = grow ? popc(nq) <= t : nq == f.Q;
+                if (fits && (f.take.empty() || cost + op.cost <= budget)) {
+                    f.Q = nq;
                     cost += op.cost;
-                    take.push_back(rem[i]);
+                    f.take.push_back(rem[i]);
                 } else {
                     blocked |= tq;
-                    defer.push_back(rem[i]);
+                    f.defer.push_back(rem[i]);
                 }
             }
             if ((blocked & all) == all) {
@@ -725,8 +735,29 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 break;
             }
         }
-        for (; i < rem.size(); ++i) defer.push_back(rem[i]);
-        rem.swap(defer);
+        for (; i < rem.size(); ++i) f.defer.push_back(rem[i]);
+        f.work = cost + 0.01f * float(f.take.size());
+        return f;
+    };
+
+    while (!rem.empty()) {
+        // Candidate tiles: the program-order greedy one, and every window of
+        // t - low consecutive qubits above the low ones.  The greedy tile
+        // follows the oldest deferred ops, which on nearest-neighbour
+        // circuits are spread over the whole register at different depths
+        // (each pass then advances scattered pairs by a layer); a window
+        // advances one block by many layers.  The tile that takes the most
+        // work wins.
+        Formed best = form(lowmask, true);
+        const int width = t - low;
+        for (int a = low; a + width <= n; ++a) {
+            const uint64_t win = (width >= 64 ? ~uint64_t(0) : ((uint64_t(1) << width) - 1)) << a;
+            Formed f = form(lowmask | win, false);
+            if (f.work > best.work) best = std::move(f);
+        }
+        uint64_t Q = best.Q;
+        std::vector<int> take = std::move(best.take);
+        rem.swap(best.defer);
 
         // Pad Q to exactly t qubits with the lowest unused ones.
         for (int q = 0; q < n && popc(Q) < t; ++q) Q |= uint64_t(1) << q;
