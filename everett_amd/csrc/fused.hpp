@@ -41,8 +41,8 @@ enum OpKind : uint8_t {
     kDiagT = 4,  // diag(d0, d1) on global qubit tq (not a register bit) where the condition holds
     kU2 = 5,     // dense 4x4 on register bits (b0 = MSB of the gate basis, b1 = LSB), b0 > b1
     kSwap = 6,   // exchange register bits b0 > b1
-    kPhase = 7,  // multiply registers whose bit b0 is set (b0 = 4: every register) by the lane's
-                 // product of b1 conditioned phase factors (see DevOp)
+    kPhase = 7,  // multiply registers whose bit b0 is set (b0 = 4: every register) and that meet
+                 // creg by the lane's product of b1 conditioned phase factors (see DevOp)
 };
 
 constexpr uint32_t kPhaseAll = 4;     // kPhase b0: the factor applies to all 16 registers

@@ -315,9 +315,9 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
             f = pick((gfix & mk_i) == mk_i, g, f);
         }
         if (b0 == kPhaseAll) {
-            op_diagt<V>(in, out, f, 0u);
+            op_diagt<V>(in, out, f, op.creg);
         } else {
-            EV_CASE_B(op_diagr, in, out, mk<V>(1, 0), f, true, 0u);
+            EV_CASE_B(op_diagr, in, out, mk<V>(1, 0), f, true, op.creg);
         }
         break;
     }

@@ -581,52 +581,73 @@ DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const R
 // product of factors once and scales its registers once.
 std::vector<DevOp> merge_phases(const std::vector<DevOp> &in, std::vector<c128> &coef)
 {
+    // A group is keyed by its register bit b0 (kPhaseAll: every register)
+    // and a register condition creg: it scales the registers j with bit b0
+    // set and (j & creg) == creg.  A controlled phase with its control in a
+    // register and its target outside is such a term on the control's bit.
     struct Group {
         size_t at;  // position in out
+        uint32_t b0, creg;
         std::vector<std::pair<uint64_t, c128>> terms;
-        DevOp first;
     };
     std::vector<DevOp> out;
     std::vector<Group> groups;
-    int open[fused::kPhaseAll + 1];
-    std::fill(open, open + fused::kPhaseAll + 1, -1);
+    std::map<uint32_t, int> open;  // (b0 << 8 | creg) -> group
     for (const DevOp &d : in) {
         const uint32_t kind = d.code & 0xff, b0 = (d.code >> 8) & 0xff, b1 = (d.code >> 16) & 0xff;
         const bool one = ((d.code >> 24) & fused::kD0IsOne) != 0;
-        int grp = -1;
+        bool term = false;
+        uint32_t gb = 0, gc = 0;
         uint64_t mask = 0;
-        if (kind == fused::kDiagR && one && d.creg == 0) {
-            grp = int(b0);
+        if (kind == fused::kDiagR && one) {
+            term = true;
+            gb = b0;
+            gc = d.creg;
             mask = d.cglob;
-        } else if (kind == fused::kDiagT && one && d.creg == 0) {
-            grp = int(fused::kPhaseAll);
+        } else if (kind == fused::kDiagT && one) {
+            term = true;
             mask = d.cglob | (uint64_t(1) << d.tq);
-        }
-        if (grp >= 0) {
-            const c128 f{d.m[2], d.m[3]};
-            if (open[grp] < 0 || groups[open[grp]].terms.size() == fused::kMaxPhaseTerms) {
-                open[grp] = int(groups.size());
-                groups.push_back({out.size(), {}, d});
-                out.push_back(d);  // placeholder, rewritten below
+            if (d.creg == 0) {
+                gb = fused::kPhaseAll;
+            } else {
+                gb = uint32_t(__builtin_ctz(d.creg));
+                gc = d.creg & ~(1u << gb);
             }
-            groups[open[grp]].terms.push_back({mask, f});
+        }
+        if (term) {
+            const c128 f{d.m[2], d.m[3]};
+            const uint32_t key = (gb << 8) | gc;
+            auto it = open.find(key);
+            if (it == open.end() || groups[it->second].terms.size() == fused::kMaxPhaseTerms) {
+                open[key] = int(groups.size());
+                groups.push_back({out.size(), gb, gc, {}});
+                out.push_back(d);  // placeholder: rewritten below, or kept when the group stays one term
+            }
+            groups[open[key]].terms.push_back({mask, f});
             continue;
         }
-        switch (kind) {  // an op that targets a register bit closes its phase group
+        uint32_t targets = 0;  // an op that targets a register bit closes the groups that read it
+        switch (kind) {
         case fused::kU1:
         case fused::kCU1:
-        case fused::kCX: open[b0] = -1; break;
+        case fused::kCX: targets = 1u << b0; break;
         case fused::kU2:
-        case fused::kSwap: open[b0] = open[b1] = -1; break;
+        case fused::kSwap: targets = (1u << b0) | (1u << b1); break;
         default: break;
+        }
+        for (auto it = open.begin(); it != open.end();) {
+            const Group &gr = groups[it->second];
+            const uint32_t reads = gr.creg | (gr.b0 == fused::kPhaseAll ? 0u : 1u << gr.b0);
+            if (reads & targets) it = open.erase(it);
+            else ++it;
         }
         out.push_back(d);
     }
     for (const Group &gr : groups) {
         if (gr.terms.size() == 1) continue;  // the original op stays
         DevOp m{};
-        const uint32_t b0 = ((gr.first.code & 0xff) == fused::kDiagT) ? fused::kPhaseAll : (gr.first.code >> 8) & 0xff;
-        m.code = fused::op_code(fused::kPhase, b0, uint32_t(gr.terms.size()), 0);
+        m.code = fused::op_code(fused::kPhase, gr.b0, uint32_t(gr.terms.size()), 0);
+        m.creg = gr.creg;
         m.coef = uint32_t(coef.size());
         for (size_t i = 0; i < gr.terms.size(); ++i) {
             std::memcpy(&m.m[i], &gr.terms[i].first, sizeof(uint64_t));

@@ -132,7 +132,8 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
                 k, b0, b1, _ = decode(o)
                 assert k <= PHASE and o["creg"] < 16
                 if k == PHASE:
-                    assert b0 <= PHASE_ALL and 2 <= b1 <= MAX_PHASE_TERMS and o["creg"] == 0 and o["cglob"] == 0
+                    assert b0 <= PHASE_ALL and 2 <= b1 <= MAX_PHASE_TERMS and o["cglob"] == 0
+                    assert b0 == PHASE_ALL or not (int(o["creg"]) >> b0) & 1
                     for mk in phase_masks(o, b1):
                         assert int(mk) < (1 << n) and all(not (int(mk) >> q) & 1 for q in regq)
                 else:
@@ -191,7 +192,7 @@ def _apply(o, v, gfix, coef):
             mk = np.uint64(mk)
             f = np.where((gfix & mk) == mk, f * coef[int(o["coef"]) + i], f)
         for j in range(16):
-            if B == PHASE_ALL or j & (1 << B):
+            if (B == PHASE_ALL or j & (1 << B)) and condj[j]:
                 v[:, j] = f * v[:, j]
     elif k == U2:
         M = m[:16].reshape(4, 4)

@@ -46,6 +46,20 @@ def mixed_circuit(n, nops, seed):
     return c
 
 
+def phase_circuit(n, rounds, seed):
+    """Hadamard layers between runs of one- and two-control phases on random qubits."""
+    rng = np.random.default_rng(seed)
+    c = ev.Circuit(n)
+    for _ in range(rounds):
+        for q in rng.choice(n, size=n // 2, replace=False):
+            c.h(int(q))
+        for _ in range(3 * n):
+            qs = [int(x) for x in rng.choice(n, size=3, replace=False)]
+            ctrls = qs[:1] if rng.random() < 0.7 else qs[:2]
+            c.controlled(ctrls, ev.Gate1.phase(float(rng.uniform(0, 6.28))), qs[2])
+    return c
+
+
 CASES = [
     ("brick10+qft", lambda: ev.workloads.config1(10, 6), {}),
     ("brick12", lambda: ev.workloads.config0(12, 8), {"tile_qubits": 9}),
@@ -56,6 +70,9 @@ CASES = [
     ("mixed9-c64", lambda: mixed_circuit(9, 150, 3), {"precision": "c64", "tile_qubits": 9}),
     ("qft12", lambda: ev.algorithms.qft(12), {"tile_qubits": 10}),
     ("mixed13", lambda: mixed_circuit(13, 160, 4), {"tile_qubits": 12}),
+    ("phases14-t10", lambda: phase_circuit(14, 4, 5), {"tile_qubits": 10}),
+    ("phases13-t9-c64", lambda: phase_circuit(13, 3, 6), {"tile_qubits": 9, "precision": "c64"}),
+    ("qft15-t10", lambda: ev.algorithms.qft(15), {"tile_qubits": 10}),
 ]
 
 
