@@ -368,15 +368,21 @@ struct Lane {
     uint64_t gfix;
 };
 
+// Branch-free over every thread-id bit a tile can have: bits at and above
+// np are masked off (threads beyond a narrower tile alias into it), and the
+// planner zero-fills the unused entries, so the descriptor reads are two
+// wide scalar loads with no wait inside a loop.
 __device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base)
 {
+    const uint32_t tm = tid & ((1u << st->np) - 1u);
     uint32_t lb = 0;
     uint64_t g = base;
-    for (int i = 0; i < st->np; ++i)
-        if ((tid >> i) & 1u) {
-            lb |= 1u << st->p[i];
-            g |= uint64_t(1) << st->gpos[i];
-        }
+#pragma unroll
+    for (int i = 0; i < kMaxTile - kRegBits; ++i) {
+        const uint32_t b = (tm >> i) & 1u;
+        lb |= b << st->p[i];
+        g |= uint64_t(b) << st->gpos[i];
+    }
     return {swz(lb), g};
 }
 
@@ -396,7 +402,8 @@ __device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Re
 
 // The stage's ops alternate between v and w, two at a time (the next op's
 // descriptor loads are in flight while the current op computes); an odd
-// last op is followed by one copy back to v.
+// last op is followed by one copy back to v.  (Exchanging straight from w
+// instead keeps both arrays live across the exchange, which spills.)
 template <class V>
 __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V> &w, uint64_t gfix, cptr<DevOp> ops,
                                               cptr<V> coef)
@@ -424,45 +431,69 @@ __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V
 template <class R> constexpr bool kSplitExchange = true;
 template <class R> constexpr size_t kLdsSlotBytes = kSplitExchange<R> ? sizeof(R) : sizeof(Amp<R>);
 
+// LDS byte addresses as 32-bit integers (local address space).
+template <class T> using lptr = __attribute__((address_space(3))) T *;
+__device__ __forceinline__ uint32_t lds_addr(void *p)
+{
+    return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lptr<unsigned char>)(p)));
+}
+template <class T> __device__ __forceinline__ lptr<T> lds_ptr(uint32_t a) { return (lptr<T>)(uintptr_t(a)); }
+
 // LDS exchange between two stages: this thread's registers leave through
 // slots ws ^ wroff[j] and the next stage's arrive from slots rs ^ rroff[j].
+// The slot maps are linear, so byte addresses are the XOR of the shifted
+// thread and register parts; each is computed once and serves both halves
+// of a split exchange.
 template <class R>
-__device__ __forceinline__ void exchange(Regs<Amp<R>> &v, void *lds, bool active, uint32_t ws, cptr<uint32_t> wroff,
-                                         uint32_t rs, cptr<uint32_t> rroff)
+__device__ __forceinline__ void exchange(Regs<Amp<R>> &v, void *lds, bool active,
+                                         uint32_t ws, cptr<uint32_t> wroff, uint32_t rs, cptr<uint32_t> rroff)
 {
     using V = Amp<R>;
+    constexpr uint32_t sh = kLdsSlotBytes<R> == 4 ? 2 : kLdsSlotBytes<R> == 8 ? 3 : 4;
+    static_assert((1u << sh) == kLdsSlotBytes<R>, "LDS slot size");
+    // The buffer is the kernel's only LDS allocation, so its base address is
+    // 0 and base + (x ^ y) = (base + x) ^ y for every slot offset: the base
+    // and the thread's part are added once, and each register costs one XOR
+    // with its (uniform) shifted register part.
+    uint32_t wb = lds_addr(lds) + (ws << sh), rb = lds_addr(lds) + (rs << sh);
+    asm("" : "+v"(wb), "+v"(rb));  // keeps the shifts from being folded back into every address
+    uint32_t wa[kRegs], ra[kRegs];
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        wa[j] = wb ^ (wroff[j] << sh);
+        ra[j] = rb ^ (rroff[j] << sh);
+    }
     if constexpr (kSplitExchange<R>) {
-        R *buf = static_cast<R *>(lds);
-        R re[kRegs];
+        // every real part is in LDS before the first read, so the reads land
+        // in v directly
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j].x;
+            for (int j = 0; j < kRegs; ++j) *lds_ptr<R>(wa[j]) = v[j].x;
         }
         lds_barrier();
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) re[j] = buf[rs ^ rroff[j]];
+            for (int j = 0; j < kRegs; ++j) v[j].x = *lds_ptr<R>(ra[j]);
         }
         lds_barrier();  // every real part read before the imaginary parts overwrite them
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j].y;
+            for (int j = 0; j < kRegs; ++j) *lds_ptr<R>(wa[j]) = v[j].y;
         }
         lds_barrier();
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) v[j] = mk<V>(re[j], buf[rs ^ rroff[j]]);
+            for (int j = 0; j < kRegs; ++j) v[j].y = *lds_ptr<R>(ra[j]);
         }
     } else {
-        V *buf = static_cast<V *>(lds);
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) buf[ws ^ wroff[j]] = v[j];
+            for (int j = 0; j < kRegs; ++j) *lds_ptr<V>(wa[j]) = v[j];
         }
         lds_barrier();
         if (active) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) v[j] = buf[rs ^ rroff[j]];
+            for (int j = 0; j < kRegs; ++j) v[j] = *lds_ptr<V>(ra[j]);
         }
     }
 }
@@ -489,11 +520,13 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
             const Lane nl = lane_of(nx, tid, tr.base);
             uint32_t rb = nl.sb;
             if (nx->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before the next stage
+                const uint32_t tm = tid & ((1u << nx->np) - 1u);
                 rb = 0;
-                for (int i = 0; i < nx->np; ++i)
-                    if ((tid >> i) & 1u) rb ^= nx->rimg[i];
-                for (int k = 0; k < nx->naff; ++k)  // CNOTs controlled by the tile index
-                    if ((tr.base >> nx->abit[k]) & 1u) rb ^= nx->aoff[k];
+#pragma unroll
+                for (int i = 0; i < kMaxTile - kRegBits; ++i) rb ^= (0u - ((tm >> i) & 1u)) & nx->rimg[i];
+#pragma unroll
+                for (int k = 0; k < kMaxAffine; ++k)  // CNOTs controlled by the tile index (unused terms are 0)
+                    rb ^= (0u - uint32_t((tr.base >> nx->abit[k]) & 1u)) & nx->aoff[k];
             }
             // A relabelled read into this stage took other threads' slots:
             // every thread must be done reading before any overwrites them.
