@@ -228,14 +228,14 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(const
 template <class T> using cptr = const EV_CONST T *;
 template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
 
-// The fields of one op descriptor that the op bodies read, held in scalar
-// registers.  The op loop fetches the next op's view while the current op
-// runs, so the scalar-cache latency hides behind the arithmetic.
+// The header fields of one op descriptor, held in scalar registers.  The op
+// loop fetches both ops of a pair up front; an op's inline coefficients are
+// read by the arm that uses them (inline_m), since holding both ops' 2x2
+// matrices in scalar registers as well spills them to vector lanes.
 template <class V> struct OpView {
     uint32_t code, creg;
     uint64_t cglob;
     uint32_t coef, tq;
-    V m[4];
     cptr<DevOp> p;
 };
 
@@ -248,11 +248,15 @@ template <class V> __device__ __forceinline__ OpView<V> fetch_op(cptr<DevOp> p)
     o.cglob = p->cglob;
     o.coef = p->coef;
     o.tq = p->tq;
-    using F = decltype(V::x);
-    const cptr<F> f = (cptr<F>)(p->m);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o.m[k] = mk<V>(f[2 * k], f[2 * k + 1]);
     return o;
+}
+
+// Inline coefficient k of an op (a 2x2 matrix or two diagonal entries).
+template <class V> __device__ __forceinline__ V inline_m(const OpView<V> &op, int k)
+{
+    using F = decltype(V::x);
+    const cptr<F> f = (cptr<F>)(op.p->m);
+    return mk<V>(f[2 * k], f[2 * k + 1]);
 }
 
 template <class V> __device__ __forceinline__ void load_row(cptr<V> m, V *out)
@@ -271,11 +275,11 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
     const uint32_t flags = op.code >> 24;
     const bool tok = (gfix & op.cglob) == op.cglob;
     switch (kind) {
-    case kU1: EV_CASE_B(op_u1, in, out, op.m[0], op.m[1], op.m[2], op.m[3], 0u); break;
+    case kU1: EV_CASE_B(op_u1, in, out, inline_m(op, 0), inline_m(op, 1), inline_m(op, 2), inline_m(op, 3), 0u); break;
     case kCU1: {
         // a lane whose global condition fails multiplies by the identity (exact)
-        const V m0 = one_unless(tok, op.m[0]), m1 = zero_unless(tok, op.m[1]);
-        const V m2 = zero_unless(tok, op.m[2]), m3 = one_unless(tok, op.m[3]);
+        const V m0 = one_unless(tok, inline_m(op, 0)), m1 = zero_unless(tok, inline_m(op, 1));
+        const V m2 = zero_unless(tok, inline_m(op, 2)), m3 = one_unless(tok, inline_m(op, 3));
         EV_CASE_B(op_u1, in, out, m0, m1, m2, m3, op.creg);
         break;
     }
@@ -288,12 +292,12 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
         break;
     case kDiagR: {
         const bool one = (flags & kD0IsOne) != 0;
-        const V d0 = one_unless(tok, op.m[0]), d1 = one_unless(tok, op.m[1]);
+        const V d0 = one_unless(tok, inline_m(op, 0)), d1 = one_unless(tok, inline_m(op, 1));
         EV_CASE_B(op_diagr, in, out, d0, d1, one, op.creg);
         break;
     }
     case kDiagT: {
-        const V d = one_unless(tok, pick(((gfix >> op.tq) & 1) != 0, op.m[1], op.m[0]));
+        const V d = one_unless(tok, pick(((gfix >> op.tq) & 1) != 0, inline_m(op, 1), inline_m(op, 0)));
         op_diagt<V>(in, out, d, op.creg);
         break;
     }
