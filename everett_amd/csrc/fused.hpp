@@ -110,6 +110,7 @@ struct Pass {
     int32_t ncomp;            // n - t
     int32_t q[16];            // local bit -> global qubit (ascending)
     int32_t comp[40];         // tile-index bit -> global qubit (the non-tile qubits, ascending)
+    int32_t swz_a, swz_b;     // the pass's LDS slot swizzle (swz below)
 };
 
 // Rewrites the inline coefficients of ops for a c64 run (floats in the
@@ -124,11 +125,18 @@ inline void inline_coef_to_c64(DevOp *ops, size_t n)
     }
 }
 
-// LDS slot of local index l: the low 4 bits are XORed with the higher
-// nibbles, so 16 consecutive lanes land in 16 distinct 16-byte slots of a
-// 256-byte LDS row for every register mapping whose varying lane bits have
-// distinct positions mod 4.  Linear over XOR, hence swz(a | b) = swz(a) ^ swz(b)
-// for disjoint a, b.
-EV_HOST_DEVICE inline uint32_t swz(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8) ^ (l >> 12)) & 15u); }
+// LDS slot of local index l in a pass's exchanges: the low 6 bits are XORed
+// with the bits a and b places above them (b = 31 folds nothing for tiles of
+// at most 16 bits).  Each output bit depends only on itself and higher bits,
+// so the map is a bijection of [0, 2^t) for every t; it is linear over XOR,
+// so swz(a ^ b) = swz(a) ^ swz(b).  The planner picks (a, b) per pass so the
+// 32 lanes that share an LDS cycle (8-byte slots, complex128) or all 64
+// (4-byte slots, complex64) hit distinct banks in as many exchanges as it can.
+constexpr int kSwzMask = 63;
+constexpr int kSwzNone = 31;
+EV_HOST_DEVICE inline uint32_t swz(uint32_t l, uint32_t a, uint32_t b)
+{
+    return l ^ ((l >> a) & kSwzMask) ^ ((l >> b) & kSwzMask);
+}
 
 }  // namespace ev::fused

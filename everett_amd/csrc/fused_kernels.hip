@@ -338,6 +338,7 @@ template <class R> struct TileRef {
     cptr<Amp<R>> coef;
     uint64_t base;
     int t, nstages;
+    uint32_t swz_a, swz_b;  // the pass's LDS slot swizzle
 };
 
 // Inserts a zero bit at every tile qubit (ascending), which spreads the tile
@@ -361,12 +362,14 @@ __device__ __forceinline__ TileRef<R> tile_ref(Amp<R> *psi, cptr<Pass> pass, cpt
     r.base = tile_base(pass, T);
     r.t = pass->t;
     r.nstages = pass->nstages;
+    r.swz_a = pass->swz_a;
+    r.swz_b = pass->swz_b;
     return r;
 }
 
 // A thread's share of a stage mapping: its thread-id bits placed at the
-// stage's non-register positions, as a swizzled LDS slot and as global
-// index bits (tile base included).
+// stage's non-register positions, as an LDS slot (the local index under the
+// pass's swizzle) and as global index bits (tile base included).
 struct Lane {
     uint32_t sb;
     uint64_t gfix;
@@ -376,7 +379,7 @@ struct Lane {
 // np are masked off (threads beyond a narrower tile alias into it), and the
 // planner zero-fills the unused entries, so the descriptor reads are two
 // wide scalar loads with no wait inside a loop.
-__device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base)
+__device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base, uint32_t swz_a, uint32_t swz_b)
 {
     const uint32_t tm = tid & ((1u << st->np) - 1u);
     uint32_t lb = 0;
@@ -387,7 +390,7 @@ __device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t b
         lb |= b << st->p[i];
         g |= uint64_t(b) << st->gpos[i];
     }
-    return {swz(lb), g};
+    return {swz(lb, swz_a, swz_b), g};
 }
 
 // Global gather of a tile in its first stage's mapping.  The planner keeps
@@ -397,7 +400,7 @@ template <class R>
 __device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v)
 {
     const cptr<Stage> st = tr.stages;
-    const Lane ln = lane_of(st, tid, tr.base);
+    const Lane ln = lane_of(st, tid, tr.base, tr.swz_a, tr.swz_b);
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         v[j] = tr.psi[ln.gfix | st->goff[j]];
@@ -515,13 +518,13 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
     const bool active = tid < (1u << (tr.t - kRegBits));
     const int nst = tr.nstages;
     Regs<V> w;  // the op loop's second register array
-    Lane ln = lane_of(tr.stages, tid, tr.base);
+    Lane ln = lane_of(tr.stages, tid, tr.base, tr.swz_a, tr.swz_b);
     for (int s = 0; s < nst; ++s) {
         const cptr<Stage> st = tr.stages + s;
         run_stage_ops<V>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
             const cptr<Stage> nx = st + 1;
-            const Lane nl = lane_of(nx, tid, tr.base);
+            const Lane nl = lane_of(nx, tid, tr.base, tr.swz_a, tr.swz_b);
             uint32_t rb = nl.sb;
             if (nx->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before the next stage
                 const uint32_t tm = tid & ((1u << nx->np) - 1u);

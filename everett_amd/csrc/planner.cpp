@@ -19,6 +19,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -703,6 +704,86 @@ uint32_t pad_registers(uint32_t S, int t, uint32_t avoid, bool must_avoid)
     return best;
 }
 
+// Lanes of one wave that meet in one LDS cycle collide when their slots
+// share a bank: 2^(bits - rank) lanes per bank, where rank is the GF(2) rank
+// of the low `bits` slot bits that the lanes' first `bits` thread-id bits
+// produce (thread-id bit i puts v[i] into the slot).
+int bank_degree(const uint32_t *v, int nv, int bits, uint32_t a, uint32_t b)
+{
+    const int k = std::min(nv, bits);
+    uint32_t basis[6] = {};  // basis[h]: the reduced vector whose highest set bit is h
+    int rank = 0;
+    for (int i = 0; i < k; ++i) {
+        uint32_t x = fused::swz(v[i], a, b) & ((1u << bits) - 1);
+        for (int h = bits - 1; h >= 0 && x; --h) {
+            if (!((x >> h) & 1)) continue;
+            if (!basis[h]) {
+                basis[h] = x;
+                ++rank;
+                x = 0;
+            } else {
+                x ^= basis[h];
+            }
+        }
+    }
+    return 1 << (k - rank);
+}
+
+// Chooses the pass's slot swizzle and applies it to the exchange tables of
+// its stages (the planner fills them unswizzled).  Every exchange of a pass
+// shares one swizzle, so a stage without relabelling writes exactly the
+// slots it read and needs no barrier between its read and its write.  The
+// score is the bank degree of the writes and the reads of every exchange,
+// for 32 lanes per cycle (8-byte slots), then for 64 (4-byte slots).
+void pass_swizzle(Pass &ps, Stage *st)
+{
+    const int ns = ps.nstages, np = ps.t - kRegBits;
+    uint32_t best_a = 6, best_b = fused::kSwzNone;
+    if (ns > 1) {
+        std::vector<std::array<uint32_t, 6>> wr, rd;
+        for (int k = 0; k + 1 < ns; ++k) {
+            std::array<uint32_t, 6> w{}, r{};
+            for (int i = 0; i < std::min(np, 6); ++i) {
+                w[i] = 1u << st[k].p[i];
+                r[i] = st[k + 1].relabel ? st[k + 1].rimg[i] : 1u << st[k + 1].p[i];
+            }
+            wr.push_back(w);
+            rd.push_back(r);
+        }
+        // Candidates in order of how often they win on brick-wall and
+        // random-circuit plans; on those, this list scores as well as every
+        // (a, b) with 2 <= a < b <= 12.
+        static constexpr uint32_t cand[][2] = {{3, 4}, {2, 4}, {3, 5}, {2, 5}, {2, 3}, {4, 7},
+                                               {4, 5}, {3, 6}, {2, 6}, {4, 6}, {5, 6}, {6, fused::kSwzNone}};
+        long best = -1;
+        const long ideal = 2L * (ns - 1) * 1000 + 2L * (ns - 1);
+        for (const auto &c : cand) {
+            long sc = 0;
+            for (size_t k = 0; k < wr.size(); ++k)
+                sc += 1000L * (bank_degree(wr[k].data(), np, 5, c[0], c[1]) +
+                               bank_degree(rd[k].data(), np, 5, c[0], c[1])) +
+                      bank_degree(wr[k].data(), np, 6, c[0], c[1]) + bank_degree(rd[k].data(), np, 6, c[0], c[1]);
+            if (best < 0 || sc < best) {
+                best = sc;
+                best_a = c[0];
+                best_b = c[1];
+            }
+            if (best == ideal) break;
+        }
+    }
+    ps.swz_a = int32_t(best_a);
+    ps.swz_b = int32_t(best_b);
+    for (int k = 0; k < ns; ++k) {
+        Stage &s = st[k];
+        for (int j = 0; j < kRegs; ++j) {
+            s.roff[j] = fused::swz(s.roff[j], best_a, best_b);
+            s.rroff[j] = fused::swz(s.rroff[j], best_a, best_b);
+        }
+        for (int i = 0; i < fused::kMaxTile; ++i) s.rimg[i] = fused::swz(s.rimg[i], best_a, best_b);
+        for (int i = 0; i < s.naff; ++i) s.aoff[i] = fused::swz(s.aoff[i], best_a, best_b);
+    }
+}
+
 // Builds the pass list for `g` on an n-qubit state.  `coalesce` is the number
 // of low qubits whose amplitudes make one 128-byte run (3 for c128, 4 for
 // c64): the first and last stage of every pass keep their register bits off
@@ -835,7 +916,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                         off |= 1u << sp[b];
                         goff |= uint64_t(1) << ps.q[sp[b]];
                     }
-                st.roff[j] = fused::swz(off);
+                st.roff[j] = off;  // swizzled with the pass (pass_swizzle)
                 st.goff[j] = goff;
             }
             st.relabel = relabel ? 1 : 0;
@@ -849,15 +930,15 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 st.naff = int(relabel->aff.size());
                 for (size_t k2 = 0; k2 < relabel->aff.size(); ++k2) {
                     st.abit[k2] = uint32_t(relabel->aff[k2].first);
-                    st.aoff[k2] = fused::swz(relabel->aff[k2].second);
+                    st.aoff[k2] = relabel->aff[k2].second;
                 }
             }
-            for (int i = 0; i < np; ++i) st.rimg[i] = fused::swz(image(1u << st.p[i]));
+            for (int i = 0; i < np; ++i) st.rimg[i] = image(1u << st.p[i]);
             for (int j = 0; j < kRegs; ++j) {
                 uint32_t off = 0;
                 for (int b = 0; b < kRegBits; ++b)
                     if ((j >> b) & 1) off |= 1u << sp[b];
-                st.rroff[j] = fused::swz(image(off));
+                st.rroff[j] = image(off);
             }
             std::vector<DevOp> lowered;
             for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef));
@@ -965,6 +1046,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         // stage does the scatter.
         if ((lastS & lowc) || in_pending) emit_stage(pad_registers(0, t, lowc, true), {}, in_pending ? &a_in : nullptr);
         ps.nstages = int(P.stages.size()) - ps.stage_begin;
+        pass_swizzle(ps, P.stages.data() + ps.stage_begin);
         P.passes.push_back(ps);
     }
     return P;

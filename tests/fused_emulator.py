@@ -18,7 +18,7 @@ from everett_amd import _ffi
 from everett_amd.errors import check
 
 PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"), ("ncomp", "<i4"),
-                       ("q", "<i4", (16,)), ("comp", "<i4", (40,))])
+                       ("q", "<i4", (16,)), ("comp", "<i4", (40,)), ("swz_a", "<i4"), ("swz_b", "<i4")])
 STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("relabel", "<i4"),
                         ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,)),
                         ("rimg", "<u4", (12,)), ("rroff", "<u4", (16,)), ("naff", "<i4"), ("abit", "<u4", (8,)),
@@ -54,8 +54,10 @@ def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low
     return passes, stages, ops[: info.num_ops], coef[: info.num_coef], info.tile_qubits
 
 
-def swz(l):
-    return l ^ (((l >> 4) ^ (l >> 8) ^ (l >> 12)) & 15)
+def swz(l, a, b):
+    """The pass's LDS slot swizzle: the low 6 bits XORed with the bits a and b
+    places above them."""
+    return l ^ ((l >> a) & 63) ^ ((l >> b) & 63)
 
 
 def _coef_span(kind, b1=0):
@@ -89,6 +91,8 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
             assert not (regpos & lowc), "first/last stage registers must leave the coalesced low qubits to lanes"
             assert [int(x) for x in st["p"][: min(coalesce, low)]] == list(range(min(coalesce, low)))
         t = int(P["t"])
+        sa, sb = int(P["swz_a"]), int(P["swz_b"])
+        assert 1 <= sa < sb <= 31, "the swizzle folds bits from above"
         assert 8 <= t <= 12 and t <= n and P["ncomp"] == n - t
         qs = sorted(list(P["q"][:t]) + list(P["comp"][: n - t]))
         assert qs == list(range(n)), "tile and complement must partition the qubits"
@@ -104,10 +108,10 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
             for i in range(t - 4):
                 lb |= ((tid >> i) & 1) << int(st["p"][i])
                 assert P["q"][st["p"][i]] == st["gpos"][i]
-            slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+            slots = swz(lb, sa, sb)[:, None] ^ st["roff"][None, :].astype(np.int64)
             assert slots.min() >= 0 and slots.max() < (1 << t)
             assert len(np.unique(slots)) == (1 << t), "stage mapping must be a permutation of the tile"
-            rslots = _read_slots(st, tid, t, lb)
+            rslots = _read_slots(st, tid, t, lb, sa, sb)
             assert rslots.min() >= 0 and rslots.max() < (1 << t)
             assert len(np.unique(rslots)) == (1 << t), "relabelled read must be a permutation of the tile"
             if not int(st["relabel"]):
@@ -221,11 +225,11 @@ def tile_base(T, q):
     return b
 
 
-def _read_slots(st, tid, t, lb, base=0):
+def _read_slots(st, tid, t, lb, sa, sb, base=0):
     """LDS slots a stage reads: its own mapping, or through the relabelling
     tables (linear images of the thread bits, affine terms on tile-index bits)."""
     if not int(st["relabel"]):
-        return swz(lb)[:, None] ^ st["rroff"][None, :].astype(np.int64)
+        return swz(lb, sa, sb)[:, None] ^ st["rroff"][None, :].astype(np.int64)
     rb = np.zeros(len(tid), dtype=np.int64)
     for i in range(t - 4):
         rb ^= ((tid >> i) & 1) * int(st["rimg"][i])
@@ -254,6 +258,7 @@ def emulate(psi, n, passes, stages, ops, coef):
         q = [int(x) for x in P["q"][:t]]
         comp = [int(x) for x in P["comp"][: n - t]]
         nst = int(P["nstages"])
+        sa, sb = int(P["swz_a"]), int(P["swz_b"])
         for T in range(1 << (n - t)):
             base = tile_base(T, q)
             want = 0
@@ -271,12 +276,12 @@ def emulate(psi, n, passes, stages, ops, coef):
                     tile_idx = set(gidx.ravel().tolist())
                     v = psi[gidx]
                 else:
-                    v = lds[_read_slots(st, tid, t, lb, base)]
+                    v = lds[_read_slots(st, tid, t, lb, sa, sb, base)]
                     assert not np.isnan(v).any(), "a stage read an LDS slot nobody wrote"
                 for o in ops[st["op_begin"]: st["op_end"]]:
                     _apply(o, v, gfix, coef)
                 if s + 1 < nst:
-                    slots = swz(lb)[:, None] ^ st["roff"][None, :].astype(np.int64)
+                    slots = swz(lb, sa, sb)[:, None] ^ st["roff"][None, :].astype(np.int64)
                     lds[:] = np.nan
                     lds[slots] = v
                 else:
