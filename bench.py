@@ -34,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "gates/sec + HBM GB/s at 30q f64; served circuits/sec with p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
+VALU_PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X published FP64 / FP32 vector peaks
 
 
 def log(msg: str) -> None:
@@ -83,12 +84,14 @@ def run_sv_workload(circuit, n, precision, steps, warmup, plain):
     agg: dict = {}
     dev_ms = 0.0
     passes = 0
+    pass_flops = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         ev.run_into(state, circuit, 0, plain=plain, profile=True)
         rs = ev.last_run_stats()
         dev_ms += rs.device_ms
         passes += rs.passes
+        pass_flops += rs.pass_flops
         for ks in ev.last_kernel_stats():
             a = agg.setdefault(ks["name"], {"launches": 0, "total_ms": 0.0, "bytes": 0.0})
             a["launches"] += ks["launches"]
@@ -99,7 +102,8 @@ def run_sv_workload(circuit, n, precision, steps, warmup, plain):
     del state
     if abs(norm - 1.0) > 1e-6:
         raise SystemExit(f"norm drifted to {norm}")
-    return {"wall": wall, "agg": agg, "dev_ms": dev_ms, "alloc_ms": alloc_ms, "passes": passes / max(steps, 1)}
+    return {"wall": wall, "agg": agg, "dev_ms": dev_ms, "alloc_ms": alloc_ms, "passes": passes / max(steps, 1),
+            "pass_flops": pass_flops}
 
 
 def roofline(agg, steps, workload_key, n, precision):
@@ -127,6 +131,24 @@ def roofline(agg, steps, workload_key, n, precision):
     }
 
 
+def compute_roofline(agg, pass_flops, precision):
+    """The fused pass against the FP64 (complex128) or FP32 vector peak: its
+    algorithmic flops per launch (each op's complex arithmetic over the
+    amplitudes it touches, summed by the library) over its average launch
+    time.  The pass is vector-ALU bound once its ops outweigh its HBM
+    traffic, which the HBM roofline alone does not show."""
+    fp = agg.get("fused_pass")
+    if not fp or not fp["launches"] or not pass_flops:
+        return None
+    per_launch = pass_flops / fp["launches"]
+    avg_ms = fp["total_ms"] / fp["launches"]
+    peak = VALU_PEAK_TFLOPS["c64" if precision == "c64" else "c128"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e12
+    return {"bound": "valu-" + ("fp32" if precision == "c64" else "fp64"), "kernel": "fused_pass",
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "flops_per_launch": per_launch}
+
+
 def bench_sv(name, circuit, n, desc, args, steps, warmup):
     r = run_sv_workload(circuit, n, args.precision, steps, warmup, args.plain)
     gates = len(circuit)
@@ -140,6 +162,7 @@ def bench_sv(name, circuit, n, desc, args, steps, warmup):
         "hbm_gbps": step_bytes / (r["wall"] / steps) / 1e9,
         "state_alloc_ms": r["alloc_ms"],
         "roofline": roofline(r["agg"], steps, name, n, args.precision),
+        "compute_roofline": compute_roofline(r["agg"], r["pass_flops"], args.precision),
         "kernels": {k: {"launches_per_step": v["launches"] / steps, "ms_per_step": v["total_ms"] / steps,
                         "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["total_ms"] else None}
                     for k, v in sorted(r["agg"].items(), key=lambda kv: -kv[1]["total_ms"])},
@@ -364,11 +387,13 @@ def main() -> int:
             "passes_per_step": r["passes_per_step"],
             "state_alloc_ms": r["state_alloc_ms"],
             "roofline": dict(r["roofline"], frac_of_copy=r["roofline"]["achieved"] / copy_gbps),
+            "compute_roofline": r["compute_roofline"],
             "kernels": r["kernels"],
             "cpu_baseline": None,
         })
+        cr = r["compute_roofline"]
         log(f"c1: {r['gates_per_s']:.0f} gates/s, {r['ms_per_step']:.1f} ms/step, roofline "
-            f"{r['roofline']['achieved']:.0f} GB/s")
+            f"{r['roofline']['achieved']:.0f} GB/s" + (f", {cr['achieved']:.1f} TFLOP/s" if cr else ""))
         if not args.no_cpu:
             log(f"c1 cpu baseline ({args.cpu_seconds:.0f} s sample)")
             out["cpu_baseline"] = cpu_baseline(c1, n1, args.cpu_seconds)

@@ -86,6 +86,7 @@ class EvRunStats(C.Structure):
         ("pass_ms", C.c_double),
         ("pass_launches", C.c_uint64),
         ("plan_ms", C.c_double),
+        ("pass_flops", C.c_double),
     ]
 
 

@@ -739,6 +739,7 @@ int ev_last_run_stats(ev_run_stats *out)
         out->pass_ms = s.pass_ms;
         out->pass_launches = s.pass_launches;
         out->plan_ms = s.plan_ms;
+        out->pass_flops = s.pass_flops;
     });
 }
 

@@ -90,6 +90,7 @@ class Profiler {
 struct RunStats {
     uint64_t logical_gates = 0, passes = 0, launches = 0, pass_launches = 0;
     double device_ms = 0, pass_ms = 0, plan_ms = 0;
+    double pass_flops = 0;  // algorithmic flops of the fused pass launches
     std::vector<KernelStat> kernels;
 };
 RunStats &last_stats();

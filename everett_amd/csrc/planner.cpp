@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -704,6 +705,28 @@ uint32_t pad_registers(uint32_t S, int t, uint32_t avoid, bool must_avoid)
     return best;
 }
 
+// Algorithmic floating-point operations per amplitude of the state that one
+// device op needs by its definition: an n-term complex dot product is 8n - 2
+// flops (an FMA counts 2), a complex scale 6, register moves none; an op
+// conditioned on register or global bits touches 2^-(condition bits) of the
+// amplitudes.
+double op_flops_per_amp(const DevOp &d)
+{
+    const uint32_t kind = d.code & 0xff, b0 = (d.code >> 8) & 0xff, flags = d.code >> 24;
+    double f = 0, share = 1;
+    switch (kind) {
+    case fused::kU1:
+    case fused::kCU1: f = 14; break;
+    case fused::kU2: f = 30; break;
+    case fused::kDiagR: f = (flags & fused::kD0IsOne) ? 3 : 6; break;
+    case fused::kDiagT: f = 6; break;
+    case fused::kPhase: f = b0 == fused::kPhaseAll ? 6 : 3; break;
+    default: return 0;
+    }
+    share = std::ldexp(1.0, -(__builtin_popcount(d.creg) + __builtin_popcountll(d.cglob)));
+    return f * share;
+}
+
 // Lanes of one wave that meet in one LDS cycle collide when their slots
 // share a bank: 2^(bits - rank) lanes per bank, where rank is the GF(2) rank
 // of the low `bits` slot bits that the lanes' first `bits` thread-id bits
@@ -1127,6 +1150,13 @@ void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
         prof.begin();
         fused::launch_pass(s->psi, c64, s->n, P.passes[p].t, dp + p, ds, dops, dcoef, s->stream);
         prof.end("fused_pass", full_pass_bytes(s->n, ab));
+        const Pass &ps = P.passes[p];
+        double fpa = 0;
+        for (int k = 0; k < ps.nstages; ++k) {
+            const Stage &sg = P.stages[size_t(ps.stage_begin + k)];
+            for (int o = sg.op_begin; o < sg.op_end; ++o) fpa += op_flops_per_amp(P.devops[size_t(o)]);
+        }
+        st.pass_flops += fpa * std::ldexp(1.0, s->n);
         ++st.passes;
         ++st.pass_launches;
         ++st.launches;

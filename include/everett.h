@@ -369,6 +369,10 @@ typedef struct {
     double pass_ms;         /* summed device time of the fused pass kernels */
     uint64_t pass_launches; /* fused pass kernel launches */
     double plan_ms;         /* host planning time */
+    double pass_flops;      /* algorithmic flops of the fused pass launches: per op, its
+                               definition's complex arithmetic (8n - 2 flops for an n-term
+                               complex dot, 6 for a complex scale) over the amplitudes it
+                               touches; no counterpart in the reference */
 } ev_run_stats;
 int ev_last_run_stats(ev_run_stats *out);
 
