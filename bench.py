@@ -15,7 +15,7 @@ restatement of the reference's loops) on a bounded prefix of the same circuit.
 Extra workloads in the same JSON line (see docs/configs.md):
   c0  20-qubit brick-wall depth 20, complex128
   c2  13-qubit noisy density matrix, depth 30, readout error 1%
-  c3  4096 uneven statevectors (10..22 qubits), complex64, one launch per round
+  c3  4096 uneven statevectors (10..22 qubits), complex64, pipelined groups, one launch per round
   c4  Poisson-arrival serving, 12..28 qubits, 1024 shots each, p50/p99 latency
 """
 from __future__ import annotations
@@ -226,7 +226,8 @@ def bench_c3(args) -> dict:
     del ref, x
     return {
         "workload": f"{count} uneven brick-wall circuits (n 10..22, depth 10..60, seeds 1000+i), complex64 bulk, "
-                    "f64 norms and renormalisation, one persistent launch per round, HIP graph",
+                    "f64 norms and renormalisation, pipelined groups (planned on host threads while the previous group runs), "
+                    "one launch per round per group",
         "circuits": count,
         "gates": gates,
         "circuits_per_s": count / wall,

@@ -1,10 +1,11 @@
 // Batched execution of many independent statevector circuits.
 //
-// All states live in one HBM arena, packed back to back.  Each circuit is
-// planned into fused passes (in parallel on host threads); round r is ONE
-// persistent launch whose workgroups stride over every tile of the r-th pass
-// of every circuit that has one, so small and large states share the GPU.
-// The rounds can be captured into a HIP graph and launched once.
+// All states live in one HBM arena, packed back to back.  Circuits run in
+// pipelined groups: each group is planned into fused passes (in parallel on
+// host threads) while the device runs the previous group; round r of a
+// group is ONE launch over every tile of the r-th pass of every circuit of
+// the group that has one, so small and large states share the GPU.  A
+// one-group batch can run its rounds as one captured HIP graph.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -42,15 +43,46 @@ struct PlanLoc {
 void batch_free(ev_batch *b)
 {
     if (!b) return;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        for (void *d : b->descs) (void)hipFreeAsync(d, b->stream);  // stream-ordered allocations
+        (void)hipStreamSynchronize(b->stream);
+    }
     if (b->arena) (void)hipFree(b->arena);
-    if (b->desc) (void)hipFree(b->desc);
     if (b->d_meta) (void)hipFree(b->d_meta);
     if (b->d_part) (void)hipFree(b->d_part);
     if (b->h_out) (void)hipHostFree(b->h_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
+
+// Runs fn(i) for i in [begin, end) on up to 16 host threads.
+template <class F> void parallel_for(size_t begin, size_t end, F fn)
+{
+    std::atomic<size_t> next{begin};
+    const unsigned nt = unsigned(std::max<size_t>(1, std::min<size_t>({16, std::thread::hardware_concurrency(), end - begin})));
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> errs(nt);
+    for (unsigned w = 0; w < nt; ++w)
+        pool.emplace_back([&, w] {
+            try {
+                for (size_t i; (i = next.fetch_add(1)) < end;) fn(i);
+            } catch (...) {
+                errs[w] = std::current_exception();
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+// The circuits run in groups of consecutive indices, pipelined: while the
+// device runs group g's rounds, the host plans group g+1 and stages its
+// descriptors in pinned memory, so only the first group's planning and
+// upload are exposed.  A group holds about kGroupAmps amplitudes (enough
+// tiles to fill the device in its early rounds); at most kMaxGroups.
+constexpr uint64_t kGroupAmps = uint64_t(1) << 28;
+constexpr size_t kMaxGroups = 8;
 
 ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options &opt)
 {
@@ -60,6 +92,21 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
     const size_t ab = c64 ? 8 : 16;
     ev_batch *b = new ev_batch;
     b->prec = c64 ? kern::C64 : kern::C128;
+    std::vector<void *> pinned;
+    std::vector<hipEvent_t> events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    auto cleanup = [&] {
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        for (void *h : pinned) (void)hipHostFree(h);
+        pinned.clear();
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        events.clear();
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    };
     try {
         // ---- validate and size
         b->n.resize(count);
@@ -82,148 +129,147 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
         b->aoff[count] = amps;
         b->total_amps = amps;
 
-        // ---- plan every circuit on host threads
-        const double t_plan = now_ms();
-        std::vector<Plan> plans(count);
-        {
-            std::atomic<size_t> next{0};
-            const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-            std::vector<std::thread> pool;
-            std::vector<std::exception_ptr> errs(nt);
-            for (unsigned w = 0; w < nt; ++w)
-                pool.emplace_back([&, w] {
-                    try {
-                        for (size_t i; (i = next.fetch_add(1)) < count;) {
-                            const FusedConfig cfg = fused_config(b->n[i], c64, opt);
-                            plans[i] = plan_circuit(*cs[i], b->n[i], cfg);
-                        }
-                    } catch (...) {
-                        errs[w] = std::current_exception();
-                    }
-                });
-            for (auto &th : pool) th.join();
-            for (auto &e : errs)
-                if (e) std::rethrow_exception(e);
+        // ---- groups of consecutive circuits with about equal amplitudes
+        const size_t ngroups = size_t(std::max<uint64_t>(1, std::min<uint64_t>({kMaxGroups, count, amps / kGroupAmps})));
+        std::vector<size_t> gbeg(ngroups + 1, count);
+        gbeg[0] = 0;
+        for (size_t g = 1, i = 0; g < ngroups; ++g) {
+            const uint64_t want = amps * g / ngroups;
+            while (i < count && b->aoff[i] < want) ++i;
+            gbeg[g] = std::max(i, gbeg[g - 1] + 1);
         }
-        b->stats.plan_ms = now_ms() - t_plan;
+        for (size_t g = 1; g <= ngroups; ++g) gbeg[g] = std::min(std::max(gbeg[g], gbeg[g - 1]), count);
 
-        // ---- descriptor layout: per-circuit plans, then the round tables
-        std::vector<PlanLoc> loc(count);
-        size_t off = 0;
-        uint32_t max_passes = 0;
-        for (size_t i = 0; i < count; ++i) {
-            const Plan &P = plans[i];
-            loc[i].pass_off = off;
-            off += align256(P.passes.size() * sizeof(fused::Pass));
-            loc[i].stage_off = off;
-            off += align256(P.stages.size() * sizeof(fused::Stage));
-            loc[i].op_off = off;
-            off += align256(std::max<size_t>(1, P.devops.size()) * sizeof(fused::DevOp));
-            loc[i].coef_off = off;
-            off += align256(std::max<size_t>(1, P.coef.size()) * ab);
-            loc[i].npass = uint32_t(P.passes.size());
-            max_passes = std::max(max_passes, loc[i].npass);
-            b->stats.passes += P.passes.size();
-        }
-        const size_t entries_off = off;
-        std::vector<size_t> round_off(max_passes), round_n(max_passes);
-        std::vector<uint64_t> round_tiles(max_passes);
-        std::vector<int> round_t(max_passes, fused::kMinTile);
-        size_t nent = 0;
-        for (uint32_t r = 0; r < max_passes; ++r) {
-            round_off[r] = nent;
-            for (size_t i = 0; i < count; ++i)
-                if (loc[i].npass > r) ++nent;
-            round_n[r] = nent - round_off[r];
-        }
-        off += align256(std::max<size_t>(1, nent) * sizeof(fused::BatchEntry));
-        const size_t desc_bytes = off;
-
-        // ---- device memory
+        // ---- device memory, and every state set to |0..0>
         EV_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         EV_HIP(hipMalloc(&b->arena, size_t(amps) * ab));
-        EV_HIP(hipMalloc(&b->desc, desc_bytes));
         EV_HIP(hipMalloc(&b->d_meta, sizeof(uint64_t) * (2 * (count + 1))));
         const uint64_t nchunks = amps / batchk::kChunk;
         EV_HIP(hipMalloc(&b->d_part, sizeof(double) * (5 * nchunks + 5 * count + count)));
         EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->h_out), sizeof(double) * 5 * count, hipHostMallocDefault));
-
-        // ---- host image of the descriptors, with device pointers in the entries
-        const double t_up = now_ms();
-        std::vector<unsigned char> host(desc_bytes, 0);
-        unsigned char *dbase = static_cast<unsigned char *>(b->desc);
-        char *abase = static_cast<char *>(b->arena);
-        for (size_t i = 0; i < count; ++i) {
-            const Plan &P = plans[i];
-            std::memcpy(host.data() + loc[i].pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
-            std::memcpy(host.data() + loc[i].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
-            if (!P.devops.empty())
-                std::memcpy(host.data() + loc[i].op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
-            if (c64)
-                fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host.data() + loc[i].op_off), P.devops.size());
-            if (!c64) {
-                std::memcpy(host.data() + loc[i].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
-            } else {
-                float *f = reinterpret_cast<float *>(host.data() + loc[i].coef_off);
-                for (size_t k = 0; k < P.coef.size(); ++k) {
-                    f[2 * k] = float(P.coef[k].re);
-                    f[2 * k + 1] = float(P.coef[k].im);
-                }
+        {
+            std::vector<uint64_t> meta(2 * (count + 1));
+            for (size_t i = 0; i <= count; ++i) {
+                meta[i] = b->aoff[i];
+                meta[count + 1 + i] = b->aoff[i] / batchk::kChunk;
             }
+            EV_HIP(hipMemcpy(b->d_meta, meta.data(), meta.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
         }
-        fused::BatchEntry *ent = reinterpret_cast<fused::BatchEntry *>(host.data() + entries_off);
-        for (uint32_t r = 0; r < max_passes; ++r) {
-            uint64_t tiles = 0;
-            size_t k = round_off[r];
-            for (size_t i = 0; i < count; ++i) {
-                if (loc[i].npass <= r) continue;
-                const fused::Pass &ps = plans[i].passes[r];
-                fused::BatchEntry &e = ent[k++];
-                e.psi = abase + b->aoff[i] * ab;
-                e.pass = reinterpret_cast<const fused::Pass *>(dbase + loc[i].pass_off) + r;
-                e.stages = reinterpret_cast<const fused::Stage *>(dbase + loc[i].stage_off);
-                e.ops = reinterpret_cast<const fused::DevOp *>(dbase + loc[i].op_off);
-                e.coef = dbase + loc[i].coef_off;
-                e.tile_begin = tiles;
-                tiles += uint64_t(1) << (b->n[i] - ps.t);
-                round_t[r] = std::max(round_t[r], int(ps.t));
-            }
-            round_tiles[r] = tiles;
-            b->stats.tiles += tiles;
-        }
-        plans.clear();
-        plans.shrink_to_fit();
-        std::vector<uint64_t> meta(2 * (count + 1));
-        for (size_t i = 0; i <= count; ++i) {
-            meta[i] = b->aoff[i];
-            meta[count + 1 + i] = b->aoff[i] / batchk::kChunk;
-        }
-        EV_HIP(hipMemcpyAsync(b->desc, host.data(), desc_bytes, hipMemcpyHostToDevice, b->stream));
-        EV_HIP(hipMemcpyAsync(b->d_meta, meta.data(), meta.size() * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
         batchk::init(b->arena, c64, amps, b->d_meta, int(count), b->stream);
-        EV_HIP(hipStreamSynchronize(b->stream));
-        b->stats.upload_ms = now_ms() - t_up;
 
-        // ---- rounds (optionally as one captured graph)
-        const fused::BatchEntry *dent = reinterpret_cast<const fused::BatchEntry *>(dbase + entries_off);
-        auto enqueue = [&] {
-            for (uint32_t r = 0; r < max_passes; ++r) {
-                fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], round_t[r], b->stream);
+        char *abase = static_cast<char *>(b->arena);
+        const bool graph_ok = opt.use_graph && ngroups == 1;
+        for (size_t g = 0; g < ngroups; ++g) {
+            const size_t i0 = gbeg[g], i1 = gbeg[g + 1], gc = i1 - i0;
+            if (gc == 0) continue;
+
+            // ---- plan the group's circuits on host threads
+            const double t_plan = now_ms();
+            std::vector<Plan> plans(gc);
+            parallel_for(0, gc, [&](size_t k) {
+                const size_t i = i0 + k;
+                plans[k] = plan_circuit(*cs[i], b->n[i], fused_config(b->n[i], c64, opt));
+            });
+            b->stats.plan_ms += now_ms() - t_plan;
+
+            // ---- descriptor layout: per-circuit plans, then the round tables
+            const double t_up = now_ms();
+            std::vector<PlanLoc> loc(gc);
+            size_t off = 0;
+            uint32_t max_passes = 0;
+            for (size_t k = 0; k < gc; ++k) {
+                const Plan &P = plans[k];
+                loc[k].pass_off = off;
+                off += align256(P.passes.size() * sizeof(fused::Pass));
+                loc[k].stage_off = off;
+                off += align256(P.stages.size() * sizeof(fused::Stage));
+                loc[k].op_off = off;
+                off += align256(std::max<size_t>(1, P.devops.size()) * sizeof(fused::DevOp));
+                loc[k].coef_off = off;
+                off += align256(std::max<size_t>(1, P.coef.size()) * ab);
+                loc[k].npass = uint32_t(P.passes.size());
+                max_passes = std::max(max_passes, loc[k].npass);
+                b->stats.passes += P.passes.size();
             }
-        };
-        hipEvent_t e0, e1;
-        EV_HIP(hipEventCreate(&e0));
-        EV_HIP(hipEventCreate(&e1));
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t gexec = nullptr;
-        auto cleanup_graph = [&] {
-            if (gexec) (void)hipGraphExecDestroy(gexec);
-            if (graph) (void)hipGraphDestroy(graph);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        };
-        try {
-            if (opt.use_graph) {
+            const size_t entries_off = off;
+            std::vector<size_t> round_off(max_passes), round_n(max_passes);
+            std::vector<uint64_t> round_tiles(max_passes);
+            std::vector<int> round_t(max_passes, fused::kMinTile);
+            size_t nent = 0;
+            for (uint32_t r = 0; r < max_passes; ++r) {
+                round_off[r] = nent;
+                for (size_t k = 0; k < gc; ++k)
+                    if (loc[k].npass > r) ++nent;
+                round_n[r] = nent - round_off[r];
+            }
+            off += align256(std::max<size_t>(1, nent) * sizeof(fused::BatchEntry));
+            const size_t desc_bytes = off;
+
+            // ---- pinned host image of the descriptors (device pointers in the
+            // entries); it stays alive until the stream has copied it
+            void *dev = nullptr;
+            EV_HIP(hipMallocAsync(&dev, desc_bytes, b->stream));
+            b->descs.push_back(dev);
+            unsigned char *host = nullptr;
+            EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), desc_bytes, hipHostMallocDefault));
+            pinned.push_back(host);
+            unsigned char *dbase = static_cast<unsigned char *>(dev);
+            parallel_for(0, gc, [&](size_t k) {
+                const Plan &P = plans[k];
+                std::memcpy(host + loc[k].pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
+                std::memcpy(host + loc[k].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
+                if (!P.devops.empty())
+                    std::memcpy(host + loc[k].op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
+                if (c64)
+                    fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host + loc[k].op_off), P.devops.size());
+                if (!c64) {
+                    std::memcpy(host + loc[k].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
+                } else {
+                    float *f = reinterpret_cast<float *>(host + loc[k].coef_off);
+                    for (size_t j = 0; j < P.coef.size(); ++j) {
+                        f[2 * j] = float(P.coef[j].re);
+                        f[2 * j + 1] = float(P.coef[j].im);
+                    }
+                }
+            });
+            fused::BatchEntry *ent = reinterpret_cast<fused::BatchEntry *>(host + entries_off);
+            for (uint32_t r = 0; r < max_passes; ++r) {
+                uint64_t tiles = 0;
+                size_t e = round_off[r];
+                for (size_t k = 0; k < gc; ++k) {
+                    if (loc[k].npass <= r) continue;
+                    const size_t i = i0 + k;
+                    const fused::Pass &ps = plans[k].passes[r];
+                    fused::BatchEntry &en = ent[e++];
+                    en.psi = abase + b->aoff[i] * ab;
+                    en.pass = reinterpret_cast<const fused::Pass *>(dbase + loc[k].pass_off) + r;
+                    en.stages = reinterpret_cast<const fused::Stage *>(dbase + loc[k].stage_off);
+                    en.ops = reinterpret_cast<const fused::DevOp *>(dbase + loc[k].op_off);
+                    en.coef = dbase + loc[k].coef_off;
+                    en.tile_begin = tiles;
+                    tiles += uint64_t(1) << (b->n[i] - ps.t);
+                    round_t[r] = std::max(round_t[r], int(ps.t));
+                }
+                round_tiles[r] = tiles;
+                b->stats.tiles += tiles;
+            }
+            plans.clear();
+            EV_HIP(hipMemcpyAsync(dev, host, desc_bytes, hipMemcpyHostToDevice, b->stream));
+            b->stats.upload_ms += now_ms() - t_up;
+
+            // ---- the group's rounds (one captured graph when the batch is one group)
+            const fused::BatchEntry *dent = reinterpret_cast<const fused::BatchEntry *>(dbase + entries_off);
+            auto enqueue = [&] {
+                for (uint32_t r = 0; r < max_passes; ++r)
+                    fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], round_t[r], b->stream);
+            };
+            hipEvent_t e0, e1;
+            EV_HIP(hipEventCreate(&e0));
+            events.push_back(e0);
+            EV_HIP(hipEventCreate(&e1));
+            events.push_back(e1);
+            if (graph_ok) {
+                EV_HIP(hipStreamSynchronize(b->stream));  // capture starts on an idle stream
                 EV_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 enqueue();
                 EV_HIP(hipStreamEndCapture(b->stream, &graph));
@@ -234,16 +280,16 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             if (gexec) EV_HIP(hipGraphLaunch(gexec, b->stream));
             else enqueue();
             EV_HIP(hipEventRecord(e1, b->stream));
-            EV_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            EV_HIP(hipEventElapsedTime(&ms, e0, e1));
-            b->stats.device_ms = ms;
-        } catch (...) {
-            cleanup_graph();
-            throw;
+            b->stats.rounds += max_passes;
+            ++b->stats.groups;
         }
-        cleanup_graph();
-        b->stats.rounds = max_passes;
+        EV_HIP(hipStreamSynchronize(b->stream));
+        for (size_t k = 0; k + 1 < events.size(); k += 2) {
+            float ms = 0;
+            EV_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
+            b->stats.device_ms += ms;
+        }
+        cleanup();
         b->stats.circuits = count;
         for (auto g : b->gates) b->stats.logical_gates += g;
 
@@ -260,6 +306,7 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             b->stats.renorm_ms = now_ms() - t_rn;
         }
     } catch (...) {
+        cleanup();
         batch_free(b);
         throw;
     }

@@ -17,7 +17,7 @@ struct ev_batch {
     std::vector<uint64_t> gates;    // logical gates of circuit i
     uint64_t total_amps = 0;
     void *arena = nullptr;          // all states, back to back
-    void *desc = nullptr;           // pass descriptors and round tables
+    std::vector<void *> descs;      // per group: pass descriptors and round tables
     uint64_t *d_meta = nullptr;     // aoff[] then chunk_begin[] on the device
     double *d_part = nullptr;       // reduction scratch
     double *h_out = nullptr;        // pinned results

@@ -172,7 +172,8 @@ typedef struct {
     uint32_t tile_qubits;   /* fused pass tile width; 0 = default */
     uint32_t max_gates_per_pass; /* 0 = default */
     uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
-    uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once */
+    uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once
+                               (batches small enough to run as one pipelined group) */
     uint32_t renormalize;   /* batch: rescale every state to unit f64 norm at the end (c64 drift) */
     uint32_t low_qubits;    /* lowest qubits always in a tile (contiguous rows); 0 = tuned or default */
 } ev_run_options;
@@ -250,15 +251,17 @@ int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, c
 
 /* Many independent measurement-free circuits (8..30 qubits each), each from
  * |0..0>, with all statevectors resident in one HBM arena.  Every circuit is
- * planned into fused passes; round r is one persistent launch over the r-th
- * pass of every circuit that has one (no reference counterpart: the
- * reference runs one circuit per call, src/backend/statevector.rs:77-88). */
+ * planned into fused passes.  Circuits run in up to 8 groups of consecutive
+ * indices, pipelined (the host plans group g+1 while the device runs group
+ * g); round r of a group is one launch over the r-th pass of every circuit
+ * of the group that has one (no reference counterpart: the reference runs
+ * one circuit per call, src/backend/statevector.rs:77-88). */
 typedef struct ev_batch ev_batch;
 typedef struct {
     uint64_t circuits, logical_gates, rounds, passes, tiles;
     double plan_ms, upload_ms, device_ms, renorm_ms;
-    uint32_t graph; /* 1 if the rounds ran as one captured HIP graph */
-    uint32_t pad;
+    uint32_t graph;  /* 1 if the rounds ran as one captured HIP graph (one-group batches) */
+    uint32_t groups; /* pipelined groups: the host plans group g+1 while the device runs group g */
 } ev_batch_stats;
 int ev_batch_run(const ev_circuit *const *circuits, size_t count, const ev_run_options *opt, ev_batch **out,
                  ev_error *err);

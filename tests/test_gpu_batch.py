@@ -72,3 +72,23 @@ def test_batch_rejects_measurement():
     c.h(0).measure(0, 0)
     with pytest.raises(ev.InvalidArgument):
         ev.Batch.run([c])
+
+
+def test_batch_pipelined_groups_equal_single_runs():
+    # 9 states of 26 qubits (2^29+ amplitudes in all) run as several pipelined
+    # groups; small states sit between them so groups split unevenly
+    sizes = [26, 10, 26, 26, 12, 26, 26, 26, 11, 26, 26, 26]
+    cs = [ev.workloads.brickwall(n, 6, ev.Rng.seed_from_u64(700 + i)) for i, n in enumerate(sizes)]
+    b = ev.Batch.run(cs, precision="c128", use_graph=True)
+    st = b.stats()
+    assert st["groups"] >= 2 and st["graph"] == 0 and st["circuits"] == len(cs)
+    assert np.max(np.abs(b.norms() - 1.0)) < 1e-12
+    for i, c in enumerate(cs):
+        n = c.num_qubits()
+        w = min(1 << 14, 1 << n)
+        s = ev.StateVectorBackend.run(c).state()
+        for off in (0, (1 << n) // 2 - w // 2, (1 << n) - w):
+            assert np.max(np.abs(b.amplitudes(i, off, w) - s.amplitudes_range(off, w))) < 1e-12, (i, off)
+        if n <= 12:
+            want, _ = oracle.sv_run(c.ops(), n)
+            assert np.max(np.abs(b.amplitudes(i) - want)) < 1e-10, i
