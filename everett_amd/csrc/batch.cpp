@@ -79,8 +79,9 @@ template <class F> void parallel_for(size_t begin, size_t end, F fn)
 // The circuits run in groups of consecutive indices, pipelined: while the
 // device runs group g's rounds, the host plans group g+1 and stages its
 // descriptors in pinned memory, so only the first group's planning and
-// upload are exposed.  A group holds about kGroupAmps amplitudes (enough
-// tiles to fill the device in its early rounds); at most kMaxGroups.
+// upload are exposed.  A nominal group holds max(kGroupAmps, amps /
+// kMaxGroups) amplitudes (enough tiles to fill the device in its early
+// rounds).
 constexpr uint64_t kGroupAmps = uint64_t(1) << 28;
 constexpr size_t kMaxGroups = 8;
 
@@ -129,16 +130,24 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
         b->aoff[count] = amps;
         b->total_amps = amps;
 
-        // ---- groups of consecutive circuits with about equal amplitudes
-        const size_t ngroups = size_t(std::max<uint64_t>(1, std::min<uint64_t>({kMaxGroups, count, amps / kGroupAmps})));
-        std::vector<size_t> gbeg(ngroups + 1, count);
-        gbeg[0] = 0;
-        for (size_t g = 1, i = 0; g < ngroups; ++g) {
-            const uint64_t want = amps * g / ngroups;
-            while (i < count && b->aoff[i] < want) ++i;
-            gbeg[g] = std::max(i, gbeg[g - 1] + 1);
+        // ---- groups of consecutive circuits: a quarter, then half the
+        // nominal size, then nominal (max(2^28, amps / 8) amplitudes), so
+        // the device starts after planning a small first group and the host
+        // stays ahead of it (planning a group takes about half the device
+        // time of the same circuits)
+        std::vector<size_t> gbeg{0};
+        {
+            const uint64_t nominal = std::max<uint64_t>(kGroupAmps, amps / kMaxGroups);
+            uint64_t target = 0, size = amps >= 2 * kGroupAmps ? nominal / 4 : amps;
+            for (size_t i = 0; i < count;) {
+                target = std::min<uint64_t>(amps, target + size);
+                while (i < count && b->aoff[i] < target) ++i;
+                if (i == gbeg.back()) ++i;  // at least one circuit per group
+                gbeg.push_back(i);
+                size = std::min<uint64_t>(nominal, 2 * size);
+            }
         }
-        for (size_t g = 1; g <= ngroups; ++g) gbeg[g] = std::min(std::max(gbeg[g], gbeg[g - 1]), count);
+        const size_t ngroups = gbeg.size() - 1;
 
         // ---- device memory, and every state set to |0..0>
         EV_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));

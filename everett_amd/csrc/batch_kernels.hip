@@ -56,14 +56,25 @@ template <int NV> __device__ void block_reduce(double (&v)[NV], double *out)
     __syncthreads();
 }
 
+// One wave per chunk: each lane sums 4 of the chunk's 256 |a|^2 (lane,
+// lane + 64, ...), then a fixed butterfly of shuffles combines the lanes, so
+// the partial needs no LDS or barrier and its rounding is deterministic.
 template <class R> __global__ void k_chunk_norm(const Amp<R> *a, uint64_t nchunks, double *part)
 {
-    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const double2 x = widen(a[c * kChunk + threadIdx.x]);
-        double v[1] = {__builtin_fma(x.x, x.x, x.y * x.y)};
-        double out[1];
-        block_reduce<1>(v, out);
-        if (threadIdx.x == 0) part[c] = out[0];
+    const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / 64;
+    const unsigned lane = threadIdx.x & 63;
+    for (uint64_t c = wave; c < nchunks; c += nwaves) {
+        const Amp<R> *p = a + c * kChunk + lane;
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < kChunk / 64; ++k) {
+            const double2 x = widen(p[64 * k]);
+            v += __builtin_fma(x.x, x.x, x.y * x.y);
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+        if (lane == 0) part[c] = v;
     }
 }
 
@@ -110,18 +121,41 @@ __global__ void k_combine(const double *part, int nv, int max_slot, const uint64
     }
 }
 
+// Each workgroup scales a contiguous range of chunks; the state of the
+// first chunk is found by binary search and later ones by stepping forward
+// (chunks are in state order), four chunks per iteration so four loads per
+// thread are in flight.
 template <class R>
 __global__ void k_scale_states(Amp<R> *a, uint64_t nchunks, const uint64_t *chunk_begin, int count, const double *scale)
 {
-    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        int lo = 0, hi = count - 1;  // last state with chunk_begin <= c
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) / 2;
-            if (chunk_begin[mid] <= c) lo = mid; else hi = mid - 1;
+    const uint64_t per = (nchunks + gridDim.x - 1) / gridDim.x;
+    const uint64_t c0 = uint64_t(blockIdx.x) * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
+    if (c0 >= c1) return;
+    int lo = 0, hi = count - 1;  // last state with chunk_begin <= c0
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (chunk_begin[mid] <= c0) lo = mid; else hi = mid - 1;
+    }
+    int st = lo;
+    uint64_t next = chunk_begin[st + 1];
+    double s = scale[st];
+    constexpr int U = 4;
+    for (uint64_t c = c0; c < c1; c += U) {
+        double sc[U];
+        Amp<R> x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            while (c + u >= next && st + 1 < count) {
+                ++st;
+                next = chunk_begin[st + 1];
+                s = scale[st];
+            }
+            sc[u] = s;
+            if (c + u < c1) x[u] = a[(c + u) * kChunk + threadIdx.x];
         }
-        const double s = scale[lo];
-        Amp<R> &x = a[c * kChunk + threadIdx.x];
-        x = mk<Amp<R>>(R(x.x * s), R(x.y * s));
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (c + u < c1) a[(c + u) * kChunk + threadIdx.x] = mk<Amp<R>>(R(x[u].x * sc[u]), R(x[u].y * sc[u]));
     }
 }
 
@@ -144,7 +178,7 @@ void norms(const void *arena, bool c64, uint64_t total_amps, const uint64_t *d_c
            double *out, hipStream_t st)
 {
     const uint64_t nchunks = total_amps / kChunk;
-    const dim3 g(static_cast<unsigned>(std::min<uint64_t>(nchunks, 8192)));
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>((nchunks + 3) / 4, 16384)));  // 4 waves (chunks) per group
     if (!c64) hipLaunchKernelGGL(k_chunk_norm<double>, g, dim3(kBlock), 0, st, static_cast<const double2 *>(arena), nchunks, part);
     else hipLaunchKernelGGL(k_chunk_norm<float>, g, dim3(kBlock), 0, st, static_cast<const float2 *>(arena), nchunks, part);
     hipLaunchKernelGGL(k_combine, dim3((count + 63) / 64), dim3(64), 0, st, part, 1, -1, d_chunk_begin, count, out);

@@ -251,10 +251,11 @@ int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, c
 
 /* Many independent measurement-free circuits (8..30 qubits each), each from
  * |0..0>, with all statevectors resident in one HBM arena.  Every circuit is
- * planned into fused passes.  Circuits run in up to 8 groups of consecutive
- * indices, pipelined (the host plans group g+1 while the device runs group
- * g); round r of a group is one launch over the r-th pass of every circuit
- * of the group that has one (no reference counterpart: the reference runs
+ * planned into fused passes.  Circuits run in groups of consecutive indices
+ * (a small first group, then about an eighth of the amplitudes each),
+ * pipelined: the host plans group g+1 while the device runs group g.  Round
+ * r of a group is one launch over the r-th pass of every circuit of the
+ * group that has one (no reference counterpart: the reference runs
  * one circuit per call, src/backend/statevector.rs:77-88). */
 typedef struct ev_batch ev_batch;
 typedef struct {
