@@ -273,11 +273,14 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
 {
     const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
     const uint32_t flags = op.code >> 24;
-    const bool tok = (gfix & op.cglob) == op.cglob;
+    // the lane's global condition, evaluated in the arms that have one (a
+    // dense 4x4 never does)
+    const auto gok = [&] { return (gfix & op.cglob) == op.cglob; };
     switch (kind) {
     case kU1: EV_CASE_B(op_u1, in, out, inline_m(op, 0), inline_m(op, 1), inline_m(op, 2), inline_m(op, 3), 0u); break;
     case kCU1: {
         // a lane whose global condition fails multiplies by the identity (exact)
+        const bool tok = gok();
         const V m0 = one_unless(tok, inline_m(op, 0)), m1 = zero_unless(tok, inline_m(op, 1));
         const V m2 = zero_unless(tok, inline_m(op, 2)), m3 = one_unless(tok, inline_m(op, 3));
         EV_CASE_B(op_u1, in, out, m0, m1, m2, m3, op.creg);
@@ -287,16 +290,18 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
         if (op.cglob == 0) {
             EV_CASE_B(op_x, in, out, op.creg);
         } else {
+            const bool tok = gok();
             EV_CASE_B(op_cx, in, out, op.creg, tok);
         }
         break;
     case kDiagR: {
-        const bool one = (flags & kD0IsOne) != 0;
+        const bool one = (flags & kD0IsOne) != 0, tok = gok();
         const V d0 = one_unless(tok, inline_m(op, 0)), d1 = one_unless(tok, inline_m(op, 1));
         EV_CASE_B(op_diagr, in, out, d0, d1, one, op.creg);
         break;
     }
     case kDiagT: {
+        const bool tok = gok();
         const V d = one_unless(tok, pick(((gfix >> op.tq) & 1) != 0, inline_m(op, 1), inline_m(op, 0)));
         op_diagt<V>(in, out, d, op.creg);
         break;
