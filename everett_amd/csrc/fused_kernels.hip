@@ -535,7 +535,8 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
                 const uint32_t tm = tid & ((1u << nx->np) - 1u);
                 rb = 0;
 #pragma unroll
-                for (int i = 0; i < kMaxTile - kRegBits; ++i) rb ^= (0u - ((tm >> i) & 1u)) & nx->rimg[i];
+                for (int i = 0; i < kMaxTile - kRegBits; ++i)  // bit i of tm as an all-ones / zero mask
+                    rb ^= uint32_t(int32_t(tm << (31 - i)) >> 31) & nx->rimg[i];
 #pragma unroll
                 for (int k = 0; k < kMaxAffine; ++k)  // CNOTs controlled by the tile index (unused terms are 0)
                     rb ^= (0u - uint32_t((tr.base >> nx->abit[k]) & 1u)) & nx->aoff[k];
