@@ -28,8 +28,10 @@ namespace ev::fused {
 
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;  // amplitudes per thread per stage
-constexpr int kMaxTile = 12;          // 2^12 amplitudes: 256 threads x 16 (64 KiB of LDS at c128, 32 KiB at c64)
+constexpr int kMaxTile = 13;          // 2^13 amplitudes: 512 threads x 16 (64 KiB of LDS at c128, 32 KiB at c64)
 constexpr int kMaxThreads = 1 << (kMaxTile - kRegBits);
+constexpr int kTileSlots = 12;        // per-thread-bit table entries in a Stage (t - 4 <= kTileSlots)
+static_assert(kMaxTile - kRegBits <= kTileSlots, "Stage tables hold every thread-id bit");
 constexpr int kMinTile = 8;
 constexpr int kMaxAffine = 8;         // relabelling terms conditioned on tile-index bits, per stage
 
@@ -93,9 +95,9 @@ struct Stage {
     int32_t relabel;          // the LDS read goes through rimg / rroff
     uint32_t roff[kRegs];     // swizzled LDS slot offset of register j
     uint64_t goff[kRegs];     // global index offset of register j
-    int32_t p[kMaxTile];      // thread-id bit i -> local position p[i]
-    int32_t gpos[kMaxTile];   // thread-id bit i -> global qubit
-    uint32_t rimg[kMaxTile];  // relabel: swizzled A(1 << p[i])
+    int32_t p[kTileSlots];     // thread-id bit i -> local position p[i]
+    int32_t gpos[kTileSlots];  // thread-id bit i -> global qubit
+    uint32_t rimg[kTileSlots]; // relabel: swizzled A(1 << p[i])
     uint32_t rroff[kRegs];    // relabel: swizzled A(register offset of j)
     int32_t naff;             // relabel: affine terms
     uint32_t abit[kMaxAffine];

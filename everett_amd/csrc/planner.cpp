@@ -802,7 +802,7 @@ void pass_swizzle(Pass &ps, Stage *st)
             s.roff[j] = fused::swz(s.roff[j], best_a, best_b);
             s.rroff[j] = fused::swz(s.rroff[j], best_a, best_b);
         }
-        for (int i = 0; i < fused::kMaxTile; ++i) s.rimg[i] = fused::swz(s.rimg[i], best_a, best_b);
+        for (int i = 0; i < fused::kTileSlots; ++i) s.rimg[i] = fused::swz(s.rimg[i], best_a, best_b);
         for (int i = 0; i < s.naff; ++i) s.aoff[i] = fused::swz(s.aoff[i], best_a, best_b);
     }
 }
@@ -1079,11 +1079,15 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
+// Tile width for sizes the tuning table does not cover: 256 threads x 16
+// amplitudes, the widest tile that keeps 12 waves per CU at complex128
+// (13-qubit tiles take 8-wave workgroups; the tuner picks them per size).
+constexpr int kDefaultTile = 12;
+
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 {
     FusedConfig f{};
-    const int tmax = fused::kMaxTile;  // 256 threads x 16 amplitudes
-    f.t = tmax;
+    f.t = kDefaultTile;
     f.low = c64 ? 5 : 4;  // 256-byte contiguous rows
     f.budget = 20.0f;
     ev_tune_entry te;
@@ -1095,8 +1099,8 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
     if (opt.tile_qubits) f.t = int(opt.tile_qubits);
     if (opt.low_qubits) f.low = int(opt.low_qubits);
     if (opt.max_gates_per_pass) f.budget = float(opt.max_gates_per_pass);
-    EV_REQUIRE(f.t >= fused::kMinTile && f.t <= tmax, EV_E_ARG,
-               "tile_qubits must be in [8, " + std::to_string(tmax) + "]");
+    EV_REQUIRE(f.t >= fused::kMinTile && f.t <= fused::kMaxTile, EV_E_ARG,
+               "tile_qubits must be in [8, " + std::to_string(fused::kMaxTile) + "]");
     EV_REQUIRE(f.low >= 1 && f.low <= 6, EV_E_ARG, "low_qubits must be in [1, 6]");
     f.t = std::min(f.t, n);
     f.low = std::min(f.low, f.t - 1);
@@ -1282,7 +1286,7 @@ void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng,
     // change the run time by under 8%; a 24-gate budget costs 1.7x).
     ev_run_options dopt = opt;
     if (!dopt.tile_qubits && !dopt.low_qubits && !dopt.max_gates_per_pass) {
-        dopt.tile_qubits = uint32_t(fused::kMaxTile);
+        dopt.tile_qubits = uint32_t(kDefaultTile);
         dopt.low_qubits = 2;
         dopt.max_gates_per_pass = 1u << 30;
     }

@@ -65,7 +65,10 @@ ev_tune_entry autotune(int n, bool c64, int depth)
             }
             return best_ms / double(c.size());
         };
-        const uint32_t tmax = uint32_t(std::min(n, fused::kMaxTile));
+        // 13-qubit tiles stay an explicit option: their 8-wave workgroups
+        // leave a CU one complex128 workgroup (registers), and at 30 qubits
+        // they measured slower at both precisions
+        const uint32_t tmax = uint32_t(std::min(n, 12));
         best.tile_qubits = tmax;
         best.low_qubits = c64 ? 5 : 4;
         best.max_gates_per_pass = 20;

@@ -169,7 +169,7 @@ int ev_sv_sample(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
 typedef struct {
     ev_precision precision; /* storage precision of the run */
     uint32_t plain;         /* 1: one launch per op (unfused); 0: planned fused passes */
-    uint32_t tile_qubits;   /* fused pass tile width; 0 = default */
+    uint32_t tile_qubits;   /* fused pass tile width, 8..13 (13 = 512-thread workgroups); 0 = default */
     uint32_t max_gates_per_pass; /* 0 = default */
     uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
     uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once

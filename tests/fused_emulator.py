@@ -93,7 +93,7 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
         t = int(P["t"])
         sa, sb = int(P["swz_a"]), int(P["swz_b"])
         assert 1 <= sa < sb <= 31, "the swizzle folds bits from above"
-        assert 8 <= t <= 12 and t <= n and P["ncomp"] == n - t
+        assert 8 <= t <= 13 and t <= n and P["ncomp"] == n - t
         qs = sorted(list(P["q"][:t]) + list(P["comp"][: n - t]))
         assert qs == list(range(n)), "tile and complement must partition the qubits"
         assert list(P["q"][:t]) == sorted(P["q"][:t])

@@ -92,3 +92,14 @@ def test_batch_pipelined_groups_equal_single_runs():
         if n <= 12:
             want, _ = oracle.sv_run(c.ops(), n)
             assert np.max(np.abs(b.amplitudes(i) - want)) < 1e-10, i
+
+
+def test_batch_mixed_tile_widths_match_oracle():
+    # 13-qubit tiles: workgroups of 512 threads; states below 13 qubits run
+    # narrower tiles in the same launches, their extra threads idle
+    cs = small_batch()[:8] + [ev.workloads.config1(n, 6) for n in (13, 14, 15)]
+    b = ev.Batch.run(cs, precision="c128", use_graph=False, tile_qubits=13)
+    for i, c in enumerate(cs):
+        n = c.num_qubits()
+        want, _ = oracle.sv_run(c.ops(), n)
+        assert np.max(np.abs(b.amplitudes(i) - want)) < 1e-10, i

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 TOL128 = 1e-10
 
 
-@pytest.mark.parametrize("n,depth,tile", [(8, 6, 0), (12, 10, 0), (13, 8, 9), (16, 12, 0)])
+@pytest.mark.parametrize("n,depth,tile", [(8, 6, 0), (12, 10, 0), (13, 8, 9), (16, 12, 0), (15, 10, 13), (17, 12, 13)])
 def test_fused_brickwall_qft_matches_oracle(n, depth, tile):
     c = ev.workloads.config1(n, depth)
     want, _ = oracle.sv_run(c.ops(), n)
@@ -60,11 +60,11 @@ def test_fused_with_measurement_segments():
         assert np.max(np.abs(ex.state().amplitudes() - want)) < TOL128
 
 
-@pytest.mark.parametrize("n", [12, 18])
-def test_fused_c64_fidelity(n):
+@pytest.mark.parametrize("n,tile", [(12, 0), (18, 0), (18, 13)])
+def test_fused_c64_fidelity(n, tile):
     c = ev.workloads.config1(n, 20)
     want, _ = oracle.sv_run(c.ops(), n)
-    s = ev.StateVectorBackend.run(c, precision="c64").state()
+    s = ev.StateVectorBackend.run(c, precision="c64", tile_qubits=tile).state()
     assert abs(s.norm_sqr() - 1.0) < 1e-6  # renormalised in f64 at the end of the run
     got = s.amplitudes()
     fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
@@ -142,15 +142,15 @@ def test_fused_config2_reduced_matches_oracle():
     assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10
 
 
-@pytest.mark.parametrize("precision", ["c128", "c64"])
-def test_fused_relabelled_exchanges_are_deterministic(precision):
-    # tile 12 = four waves per workgroup.  CNOTs between tile qubits, and
-    # CNOTs controlled from outside the tile, become relabellings of the LDS
-    # exchange whose reads take other threads' slots: a missing barrier would
-    # show as run-to-run differences and oracle mismatches here.
+@pytest.mark.parametrize("precision,tile", [("c128", 12), ("c64", 12), ("c128", 13), ("c64", 13)])
+def test_fused_relabelled_exchanges_are_deterministic(precision, tile):
+    # tile 12 = four waves per workgroup, tile 13 = eight.  CNOTs between tile
+    # qubits, and CNOTs controlled from outside the tile, become relabellings
+    # of the LDS exchange whose reads take other threads' slots: a missing
+    # barrier would show as run-to-run differences and oracle mismatches here.
     n = 18
     c = ev.workloads.config1(n, 30)
-    runs = [ev.StateVectorBackend.run(c, precision=precision, tile_qubits=12).state().amplitudes() for _ in range(4)]
+    runs = [ev.StateVectorBackend.run(c, precision=precision, tile_qubits=tile).state().amplitudes() for _ in range(4)]
     for r in runs[1:]:
         assert np.array_equal(r, runs[0])
     want, _ = oracle.sv_run(c.ops(), n)

@@ -210,7 +210,7 @@ def test_tune_table_roundtrip_and_plan_uses_it():
     _, _, _, _, t = get_plan(c, precision="c64", tile_qubits=11)  # explicit options win
     assert t == 11
     with pytest.raises(ev.InvalidArgument):
-        ev.tune_set(31, "c64", {"tile_qubits": 13, "low_qubits": 3, "max_gates_per_pass": 7})
+        ev.tune_set(31, "c64", {"tile_qubits": 14, "low_qubits": 3, "max_gates_per_pass": 7})
 
 
 @pytest.mark.parametrize("low", [1, 2, 3, 5, 6])

@@ -70,6 +70,7 @@ CASES = [
     ("mixed9-c64", lambda: mixed_circuit(9, 150, 3), {"precision": "c64", "tile_qubits": 9}),
     ("qft12", lambda: ev.algorithms.qft(12), {"tile_qubits": 10}),
     ("mixed13", lambda: mixed_circuit(13, 160, 4), {"tile_qubits": 12}),
+    ("mixed15_tile13", lambda: mixed_circuit(15, 220, 5), {"tile_qubits": 13}),
     ("phases14-t10", lambda: phase_circuit(14, 4, 5), {"tile_qubits": 10}),
     ("phases13-t9-c64", lambda: phase_circuit(13, 3, 6), {"tile_qubits": 9, "precision": "c64"}),
     ("qft15-t10", lambda: ev.algorithms.qft(15), {"tile_qubits": 10}),
@@ -109,11 +110,12 @@ def test_plan_structure_c64_30_qubits():
 
 
 def test_tile_option_bounds():
-    c = ev.workloads.config0(12, 2)
+    c = ev.workloads.config0(16, 2)
     with pytest.raises(ev.InvalidArgument):
-        get_plan(c, tile_qubits=13)  # 2^13 amplitudes need 512 threads, above the kernel's launch bound
+        get_plan(c, tile_qubits=14)  # 2^14 amplitudes need 1024 threads, above the kernel's launch bound
     with pytest.raises(ev.InvalidArgument):
         get_plan(c, tile_qubits=7)
+    assert get_plan(c, tile_qubits=13)[4] == 13
 
 
 def test_controlled_phases_merge_into_phase_ops():
