@@ -31,7 +31,9 @@ class Batch:
     @staticmethod
     def run(circuits, precision="c64", use_graph: bool = True, renormalize: bool = False, tile_qubits: int = 0,
             max_gates_per_pass: int = 0) -> "Batch":
-        """Runs every circuit from |0..0>; round r is one launch over the r-th pass of every circuit."""
+        """Runs every circuit from |0..0>.  Circuits run in pipelined groups (the host plans the next group
+        while the device runs the current one); round r of a group is one launch over the r-th pass of
+        every circuit in the group."""
         opt = run_options(precision, False, tile_qubits, max_gates_per_pass)
         opt.use_graph = 1 if use_graph else 0
         opt.renormalize = 1 if renormalize else 0

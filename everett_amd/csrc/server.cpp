@@ -5,8 +5,8 @@
 //   1. admits queued requests in arrival order while their statevectors fit
 //      the HBM budget: allocates the state (from a size-keyed cache of device
 //      blocks), uploads the plan, resets the state to |0..0>;
-//   2. runs one step: ONE persistent launch over the next pass of every
-//      admitted request (the batch kernel), so a 28-qubit request and a
+//   2. runs one step: ONE launch over the next pass of every admitted
+//      request (the batch kernel, a workgroup per tile), so a 28-qubit request and a
 //      handful of 12-qubit ones share each launch;
 //   3. samples requests whose last pass ran (f64 CDF, host-drawn uniforms
 //      from Rng::seed_from_u64(seed)), releases their memory and wakes waiters.

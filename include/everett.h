@@ -283,8 +283,9 @@ int ev_batch_get_stats(const ev_batch *b, ev_batch_stats *out);
  * measurement-free circuits ending in `shots` Born-rule samples.  A
  * scheduler thread admits queued requests in arrival order while their
  * statevectors fit the HBM budget (2^n * 8 B for c64, 2^n * 16 B for c128),
- * then advances every admitted request by one fused pass per persistent
- * launch, so requests of different sizes overlap on the device.  A request
+ * then advances every admitted request by one fused pass per launch (one
+ * workgroup per tile of every request's next pass), so requests of
+ * different sizes overlap on the device.  A request
  * that completes is sampled (f64 CDF, one Rng::next_f64 per shot from
  * Rng::seed_from_u64(seed)) and its memory is released.  Submit, poll and
  * wait are thread-safe. */
