@@ -59,12 +59,14 @@ class RunStats:
     pass_ms: float
     pass_launches: int
     plan_ms: float
+    pass_flops: float  # algorithmic flops of the fused pass launches
 
 
 def last_run_stats() -> RunStats:
     s = _ffi.EvRunStats()
     check(_ffi.lib().ev_last_run_stats(C.byref(s)))
-    return RunStats(s.logical_gates, s.passes, s.launches, s.device_ms, s.pass_ms, s.pass_launches, s.plan_ms)
+    return RunStats(s.logical_gates, s.passes, s.launches, s.device_ms, s.pass_ms, s.pass_launches, s.plan_ms,
+                    s.pass_flops)
 
 
 class Execution:

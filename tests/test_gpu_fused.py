@@ -158,3 +158,18 @@ def test_fused_relabelled_exchanges_are_deterministic(precision, tile):
         assert np.max(np.abs(runs[0] - want)) < TOL128
     else:
         assert np.max(np.abs(runs[0] - want)) < 1e-5
+
+
+def test_run_stats_count_algorithmic_pass_flops():
+    # one dense 4x4 on a 10-qubit state is one pass of 30 flops per amplitude
+    # (a 4-term complex dot product: 4 complex products of 6, 3 sums of 2)
+    c = ev.Circuit(10)
+    c.gate2(ev.Gate2(random_unitary(np.random.default_rng(3), 4)), 3, 7)
+    ev.StateVectorBackend.run(c)
+    rs = ev.last_run_stats()
+    assert rs.passes == 1 and rs.pass_flops == 30 * 2**10
+    # a CNOT moves amplitudes and does no arithmetic
+    c = ev.Circuit(10)
+    c.cnot(2, 6)
+    ev.StateVectorBackend.run(c)
+    assert ev.last_run_stats().pass_flops == 0
