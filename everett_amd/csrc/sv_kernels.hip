@@ -34,6 +34,18 @@ inline dim3 grid_for(uint64_t work)
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (count);         \
          i += uint64_t(gridDim.x) * blockDim.x)
 
+// The one-qubit kernels stride over their work in blocks of kU x blockDim
+// items: a thread takes items i0, i0 + S, ..., i0 + (kU - 1) S (S =
+// blockDim, so each of the kU steps is one coalesced run) and loads every
+// item of its group before storing any, so kU independent loads per thread
+// are in flight.  Measured at 30 qubits against one item per iteration on
+// the same MI355X: dense +2%, diagonal +4%; the same grouping made CNOT and
+// SWAP (half the state, strided pairs) 6-8% slower, so they keep one item.
+constexpr int kU = 4;
+#define GRID_STRIDE_GROUPS(i0, S, count)                                                        \
+    for (uint64_t i0 = uint64_t(blockIdx.x) * (kU * blockDim.x) + threadIdx.x, S = blockDim.x; \
+         i0 < (count); i0 += uint64_t(gridDim.x) * (kU * blockDim.x))
+
 template <class R> M2<R> to_m2(const c128 m[4])
 {
     M2<R> o;
@@ -58,12 +70,24 @@ template <class R> __global__ void k_init_basis(Amp<R> *psi, uint64_t len, uint6
 template <class R> __global__ void k_apply1_dense(Amp<R> *psi, uint64_t npairs, int k, M2<R> g)
 {
     const uint64_t bit = uint64_t(1) << k;
-    GRID_STRIDE(i, npairs)
+    GRID_STRIDE_GROUPS(i0, S, npairs)
     {
-        const uint64_t i0 = insert0(i, k), i1 = i0 | bit;
-        const Amp<R> a0 = psi[i0], a1 = psi[i1];
-        psi[i0] = cadd(cmul(g.m[0], a0), cmul(g.m[1], a1));
-        psi[i1] = cadd(cmul(g.m[2], a0), cmul(g.m[3], a1));
+        uint64_t j[kU];
+        Amp<R> a0[kU], a1[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            j[u] = insert0(i0 + u * S, k);
+            if (i0 + u * S < npairs) {
+                a0[u] = psi[j[u]];
+                a1[u] = psi[j[u] | bit];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * S < npairs) {
+                psi[j[u]] = cadd(cmul(g.m[0], a0[u]), cmul(g.m[1], a1[u]));
+                psi[j[u] | bit] = cadd(cmul(g.m[2], a0[u]), cmul(g.m[3], a1[u]));
+            }
     }
 }
 
@@ -72,11 +96,24 @@ template <class R> __global__ void k_apply1_dense(Amp<R> *psi, uint64_t npairs, 
 template <class R> __global__ void k_apply1_diag(Amp<R> *psi, uint64_t npairs, int k, Amp<R> d0, Amp<R> d1)
 {
     const uint64_t bit = uint64_t(1) << k;
-    GRID_STRIDE(i, npairs)
+    GRID_STRIDE_GROUPS(i0, S, npairs)
     {
-        const uint64_t i0 = insert0(i, k), i1 = i0 | bit;
-        psi[i0] = cmul(d0, psi[i0]);
-        psi[i1] = cmul(d1, psi[i1]);
+        uint64_t j[kU];
+        Amp<R> a0[kU], a1[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            j[u] = insert0(i0 + u * S, k);
+            if (i0 + u * S < npairs) {
+                a0[u] = psi[j[u]];
+                a1[u] = psi[j[u] | bit];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * S < npairs) {
+                psi[j[u]] = cmul(d0, a0[u]);
+                psi[j[u] | bit] = cmul(d1, a1[u]);
+            }
     }
 }
 
@@ -85,10 +122,18 @@ template <class R> __global__ void k_apply1_diag(Amp<R> *psi, uint64_t npairs, i
 template <class R> __global__ void k_apply1_phase(Amp<R> *psi, uint64_t npairs, int k, Amp<R> d1)
 {
     const uint64_t bit = uint64_t(1) << k;
-    GRID_STRIDE(i, npairs)
+    GRID_STRIDE_GROUPS(i0, S, npairs)
     {
-        const uint64_t i1 = insert0(i, k) | bit;
-        psi[i1] = cmul(d1, psi[i1]);
+        uint64_t j[kU];
+        Amp<R> a1[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            j[u] = insert0(i0 + u * S, k) | bit;
+            if (i0 + u * S < npairs) a1[u] = psi[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i0 + u * S < npairs) psi[j[u]] = cmul(d1, a1[u]);
     }
 }
 
