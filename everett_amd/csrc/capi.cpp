@@ -627,7 +627,7 @@ int ev_dm_trace(const ev_dm *d, double *out)
     return guard([&] {
         NONNULL(d);
         NONNULL(out);
-        *out = ev::dm_diag_sum(const_cast<ev_dm *>(d), 0);
+        *out = ev::dm_diag_sum(const_cast<ev_dm *>(d), 0, 0);
     });
 }
 

@@ -87,9 +87,9 @@ static void dfetch(ev_dm *d, int count)
     sv_sync(s);
 }
 
-double dm_diag_sum(ev_dm *d, uint64_t mask)
+double dm_diag_sum(ev_dm *d, uint64_t mask, uint64_t want)
 {
-    kern::dm_diag_sum(d->vec->psi, d->n, mask, d->vec->d_red, dslots(d), d->vec->stream);
+    kern::dm_diag_sum(d->vec->psi, d->n, mask, want, d->vec->d_red, dslots(d), d->vec->stream);
     dfetch(d, 1);
     return d->vec->h_red[0];
 }
@@ -104,18 +104,24 @@ void dm_partial_trace(ev_dm *d, int q, c128 out[4])
 // density.rs:219-227
 void dm_renormalize(ev_dm *d)
 {
-    const double t = dm_diag_sum(d, 0);
+    const double t = dm_diag_sum(d, 0, 0);
     if (t > 0.0) kern::scale(d->vec->psi, kern::C128, 2 * d->n, 1.0 / t, d->vec->stream);
 }
 
 // density_matrix.rs:155-169: p1 from the diagonal, one draw, collapse,
 // renormalise, then the readout flip (second draw only if readout_error > 0).
+// The trace after the collapse is the sum of the kept diagonal entries, taken
+// before it in the same order (a zeroed entry adds +0 to a partial sum and
+// changes nothing), so the collapse and the 1/trace scaling run as one pass.
+// For outcome 1 that sum is p1 itself.
 bool dm_measure(ev_dm *d, int q, Rng &rng, double readout_error)
 {
-    const double p1 = dm_diag_sum(d, uint64_t(1) << q);
+    const uint64_t bit = uint64_t(1) << q;
+    const double p1 = dm_diag_sum(d, bit, bit);
     const bool raw = rng.next_f64() < p1;
-    kern::dm_collapse(d->vec->psi, d->n, q, raw ? 1 : 0, d->vec->stream);
-    dm_renormalize(d);
+    const double t = raw ? p1 : dm_diag_sum(d, bit, 0);
+    if (t > 0.0) kern::dm_collapse_scale(d->vec->psi, d->n, q, raw ? 1 : 0, 1.0 / t, d->vec->stream);
+    else kern::dm_collapse(d->vec->psi, d->n, q, raw ? 1 : 0, d->vec->stream);
     if (readout_error > 0.0 && rng.next_f64() < readout_error) return !raw;
     return raw;
 }

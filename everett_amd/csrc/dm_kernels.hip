@@ -64,13 +64,13 @@ template <int NV> __global__ void k_finalize_dm(const double *partials, int nblo
 
 // density.rs:121-124 (mask = 0) and density_matrix.rs:175-186 (mask = bit q):
 // sum of Re rho[j][j] over j with (j & mask) == mask.
-__global__ void k_diag_partial(const double2 *rho, int n, uint64_t mask, double *partials)
+__global__ void k_diag_partial(const double2 *rho, int n, uint64_t mask, uint64_t want, double *partials)
 {
     double v[1] = {0.0};
     const uint64_t dim = uint64_t(1) << n;
     GRID_STRIDE(j, dim)
     {
-        if ((j & mask) == mask) v[0] += rho[(j << n) | j].x;
+        if ((j & mask) == want) v[0] += rho[(j << n) | j].x;
     }
     block_sum<1>(v, partials);
 }
@@ -84,6 +84,23 @@ __global__ void k_dm_collapse(double2 *rho, int n, int q, int outcome)
     {
         const bool ok = (((x >> (n + q)) & 1) == uint64_t(outcome)) && (((x >> q) & 1) == uint64_t(outcome));
         if (!ok) rho[x] = make_double2(0.0, 0.0);
+    }
+}
+
+// k_dm_collapse, then every entry times s (density.rs:219-227 with s = 1 /
+// trace): a zeroed entry stays +0, so the result equals the two passes'.
+__global__ void k_dm_collapse_scale(double2 *rho, int n, int q, int outcome, double s)
+{
+    const uint64_t len = uint64_t(1) << (2 * n);
+    GRID_STRIDE(x, len)
+    {
+        const bool ok = (((x >> (n + q)) & 1) == uint64_t(outcome)) && (((x >> q) & 1) == uint64_t(outcome));
+        if (ok) {
+            const double2 e = rho[x];
+            rho[x] = make_double2(e.x * s, e.y * s);
+        } else {
+            rho[x] = make_double2(0.0, 0.0);
+        }
     }
 }
 
@@ -110,11 +127,18 @@ __global__ void k_ptrace_partial(const double2 *rho, int n, int q, double *parti
 
 }  // namespace
 
-void dm_diag_sum(const void *rho, int n, uint64_t mask, double *partials, double *out, hipStream_t st)
+void dm_diag_sum(const void *rho, int n, uint64_t mask, uint64_t want, double *partials, double *out, hipStream_t st)
 {
     const dim3 g = rgrid(uint64_t(1) << n);
-    hipLaunchKernelGGL(k_diag_partial, g, dim3(kBlock), 0, st, static_cast<const double2 *>(rho), n, mask, partials);
+    hipLaunchKernelGGL(k_diag_partial, g, dim3(kBlock), 0, st, static_cast<const double2 *>(rho), n, mask, want,
+                       partials);
     hipLaunchKernelGGL(k_finalize_dm<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void dm_collapse_scale(void *rho, int n, int q, int outcome, double s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dm_collapse_scale, grid_for(uint64_t(1) << (2 * n)), dim3(kBlock), 0, st,
+                       static_cast<double2 *>(rho), n, q, outcome, s);
 }
 
 void dm_collapse(void *rho, int n, int q, int outcome, hipStream_t st)

@@ -26,7 +26,8 @@ Mat4 channel_superop(const std::vector<Mat2> &kraus);
 Mat4 conj4(const c128 *m);
 Mat2 conj2(const c128 *m);
 
-double dm_diag_sum(ev_dm *d, uint64_t mask);
+// Sum of Re rho[j][j] over j with (j & mask) == want.
+double dm_diag_sum(ev_dm *d, uint64_t mask, uint64_t want);
 void dm_partial_trace(ev_dm *d, int q, c128 out[4]);
 void dm_renormalize(ev_dm *d);
 bool dm_measure(ev_dm *d, int q, Rng &rng, double readout_error);
