@@ -126,6 +126,58 @@ bool dm_measure(ev_dm *d, int q, Rng &rng, double readout_error)
     return raw;
 }
 
+// A run of consecutive Measure ops (density_matrix.rs:155-169 for each, in
+// order).  Each measurement's probabilities come from a device copy of the
+// diagonal that is collapsed and rescaled as the matrix would be (the same
+// partial sums, so the same p1, draws and outcomes as measuring one by one),
+// and the matrix itself takes one pass for the whole run: the entries that
+// match every outcome, times the scales in order.  Returns the index after
+// the run.
+size_t dm_measure_run(ev_dm *d, const Circuit &c, size_t i, Rng &rng, double readout_error, uint8_t *classical)
+{
+    ev_sv *v = d->vec;
+    const int n = d->n;
+    const uint64_t dim = uint64_t(1) << n;
+    double *diag = nullptr;
+    EV_HIP(hipMallocAsync(reinterpret_cast<void **>(&diag), sizeof(double) * dim, v->stream));
+    try {
+        kern::dm_extract_diag(v->psi, n, diag, v->stream);
+        kern::DmScales sc{};
+        uint64_t keep_mask = 0, keep_val = 0;
+        bool none = false;
+        auto diag_sum = [&](uint64_t mask, uint64_t want) {
+            kern::diag_vec_sum(diag, n, mask, want, v->d_red, dslots(d), v->stream);
+            dfetch(d, 1);
+            return v->h_red[0];
+        };
+        while (i < c.size() && c.ops()[i].kind == OpKind::Measure && sc.count < kern::kMaxBlockMeasures) {
+            const Op &op = c.ops()[i];
+            const int q = int(op.q0);
+            const uint64_t bit = uint64_t(1) << q;
+            const double p1 = diag_sum(bit, bit);
+            const bool raw = rng.next_f64() < p1;
+            const double t = raw ? p1 : diag_sum(bit, 0);
+            const int o = raw ? 1 : 0;
+            const uint64_t both = bit | (bit << n), val = raw ? both : 0;
+            if ((keep_mask & both) && (keep_val & both) != val) none = true;  // an outcome contradicting an earlier one
+            keep_mask |= both;
+            keep_val = (keep_val & ~both) | val;
+            if (t > 0.0) sc.s[sc.count++] = 1.0 / t;
+            kern::diag_collapse(diag, n, q, o, t > 0.0 ? 1.0 / t : 1.0, t > 0.0, v->stream);
+            bool bitv = raw;
+            if (readout_error > 0.0 && rng.next_f64() < readout_error) bitv = !raw;
+            classical[op.q1] = bitv ? 1 : 0;
+            ++i;
+        }
+        kern::dm_collapse_block(v->psi, n, keep_mask, keep_val, none, sc, v->stream);
+    } catch (...) {
+        (void)hipFreeAsync(diag, v->stream);
+        throw;
+    }
+    EV_HIP(hipFreeAsync(diag, v->stream));
+    return i;
+}
+
 DmNoise make_dm_noise(const ev_noise_model *noise)
 {
     DmNoise nz;

@@ -104,6 +104,59 @@ __global__ void k_dm_collapse_scale(double2 *rho, int n, int q, int outcome, dou
     }
 }
 
+// A block of consecutive measurements works on a copy of the diagonal:
+// diag[j] = Re rho[j][j].
+__global__ void k_dm_extract_diag(const double2 *rho, int n, double *diag)
+{
+    const uint64_t dim = uint64_t(1) << n;
+    GRID_STRIDE(j, dim) diag[j] = rho[(j << n) | j].x;
+}
+
+// k_diag_partial over the diagonal copy: the same threads add the same
+// entries in the same order, so the sums equal the matrix's bit for bit.
+__global__ void k_vec_partial(const double *diag, int n, uint64_t mask, uint64_t want, double *partials)
+{
+    double v[1] = {0.0};
+    const uint64_t dim = uint64_t(1) << n;
+    GRID_STRIDE(j, dim)
+    {
+        if ((j & mask) == want) v[0] += diag[j];
+    }
+    block_sum<1>(v, partials);
+}
+
+// One measurement on the diagonal copy: entries whose bit q differs from the
+// outcome become 0, the others are multiplied by s (when scaled).
+__global__ void k_diag_collapse(double *diag, int n, int q, int outcome, double s, int scaled)
+{
+    const uint64_t dim = uint64_t(1) << n;
+    GRID_STRIDE(j, dim)
+    {
+        if (((j >> q) & 1) != uint64_t(outcome)) diag[j] = 0.0;
+        else if (scaled) diag[j] = diag[j] * s;
+    }
+}
+
+// Every measurement of a block in one pass over rho: an entry survives when
+// its row and column bits match every outcome ((x & keep_mask) == keep_val),
+// and is then multiplied by the block's scales in order (the value each
+// measurement's own collapse-and-scale pass would leave); the rest become 0.
+__global__ void k_dm_collapse_block(double2 *rho, int n, uint64_t keep_mask, uint64_t keep_val, int none,
+                                    DmScales sc)
+{
+    const uint64_t len = uint64_t(1) << (2 * n);
+    GRID_STRIDE(x, len)
+    {
+        if (!none && (x & keep_mask) == keep_val) {
+            double2 e = rho[x];
+            for (int k = 0; k < sc.count; ++k) e = make_double2(e.x * sc.s[k], e.y * sc.s[k]);
+            rho[x] = e;
+        } else {
+            rho[x] = make_double2(0.0, 0.0);
+        }
+    }
+}
+
 // density.rs:270-288: reduced[iq*2+jq] = sum over the other bits of rho[i][j].
 __global__ void k_ptrace_partial(const double2 *rho, int n, int q, double *partials)
 {
@@ -133,6 +186,32 @@ void dm_diag_sum(const void *rho, int n, uint64_t mask, uint64_t want, double *p
     hipLaunchKernelGGL(k_diag_partial, g, dim3(kBlock), 0, st, static_cast<const double2 *>(rho), n, mask, want,
                        partials);
     hipLaunchKernelGGL(k_finalize_dm<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void dm_extract_diag(const void *rho, int n, double *diag, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dm_extract_diag, grid_for(uint64_t(1) << n), dim3(kBlock), 0, st,
+                       static_cast<const double2 *>(rho), n, diag);
+}
+
+void diag_vec_sum(const double *diag, int n, uint64_t mask, uint64_t want, double *partials, double *out, hipStream_t st)
+{
+    const dim3 g = rgrid(uint64_t(1) << n);
+    hipLaunchKernelGGL(k_vec_partial, g, dim3(kBlock), 0, st, diag, n, mask, want, partials);
+    hipLaunchKernelGGL(k_finalize_dm<1>, dim3(1), dim3(1024), 0, st, partials, int(g.x), out);
+}
+
+void diag_collapse(double *diag, int n, int q, int outcome, double s, bool scaled, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_diag_collapse, grid_for(uint64_t(1) << n), dim3(kBlock), 0, st, diag, n, q, outcome, s,
+                       scaled ? 1 : 0);
+}
+
+void dm_collapse_block(void *rho, int n, uint64_t keep_mask, uint64_t keep_val, bool none, const DmScales &sc,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dm_collapse_block, grid_for(uint64_t(1) << (2 * n)), dim3(kBlock), 0, st,
+                       static_cast<double2 *>(rho), n, keep_mask, keep_val, none ? 1 : 0, sc);
 }
 
 void dm_collapse_scale(void *rho, int n, int q, int outcome, double s, hipStream_t st)

@@ -31,6 +31,9 @@ double dm_diag_sum(ev_dm *d, uint64_t mask, uint64_t want);
 void dm_partial_trace(ev_dm *d, int q, c128 out[4]);
 void dm_renormalize(ev_dm *d);
 bool dm_measure(ev_dm *d, int q, Rng &rng, double readout_error);
+// Consecutive Measure ops from index i as one pass over rho; returns the
+// index after them.
+size_t dm_measure_run(ev_dm *d, const Circuit &c, size_t i, Rng &rng, double readout_error, uint8_t *classical);
 
 // backend/mod.rs:51-72 over the density-matrix primitives for record i, one
 // launch per (super)operator; returns the index after the op.

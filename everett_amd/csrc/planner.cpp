@@ -1337,6 +1337,9 @@ void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng,
             break;
         }
         case OpKind::Measure:
+            flush_segment(v, low, cfg, st, prof);
+            i = dm_measure_run(d, c, i, rng, nz.readout_error, classical);
+            break;
         case OpKind::If:
             flush_segment(v, low, cfg, st, prof);
             i = dm_plain_op(d, c, i, nz, rng, classical, true, st);

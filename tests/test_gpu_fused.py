@@ -142,6 +142,30 @@ def test_fused_config2_reduced_matches_oracle():
     assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10
 
 
+def test_fused_dm_measurement_runs_match_oracle():
+    # consecutive measurements run as one pass over rho (a device copy of the
+    # diagonal gives each one its probabilities); runs here repeat a qubit,
+    # follow gates, and feed a conditional, with readout error on
+    n = 5
+    c = ev.Circuit.with_classical(n, 9)
+    c.compose(ev.workloads.brickwall(n, 3, ev.Rng.seed_from_u64(11)))
+    c.measure(0, 0).measure(3, 1).measure(0, 2).measure(4, 3)
+    c.x_if(1, 2)
+    c.compose(ev.workloads.brickwall(n, 2, ev.Rng.seed_from_u64(12)))
+    for k, q in enumerate((1, 2, 3, 4, 2)):
+        c.measure(q, 4 + k)
+    noise = ev.NoiseModel.uniform_depolarizing(0.05)
+    noise.readout_error = 0.2
+    ref = {"kind_1q": "depolarizing", "p_1q": 0.05, "kind_2q": "depolarizing", "p_2q": 0.05, "readout_error": 0.2}
+    for s in range(6):
+        ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, s)
+        want, bits = oracle.dm_run(c.ops(), n, ref, s, num_classical=9)
+        assert ex.classical() == bits, s
+        assert np.max(np.abs(ex.density_matrix().data() - want)) < 1e-10, s
+        plain = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, s, plain=True)
+        assert plain.classical() == bits, s
+
+
 @pytest.mark.parametrize("precision,tile", [("c128", 12), ("c64", 12), ("c128", 13), ("c64", 13)])
 def test_fused_relabelled_exchanges_are_deterministic(precision, tile):
     # tile 12 = four waves per workgroup, tile 13 = eight.  CNOTs between tile
