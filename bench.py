@@ -351,8 +351,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: every GPU box here has a single MI355X and this build has no "
-              "multi-GPU path (the sharding design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (the statevector would shard by "
+              "its top qubits, one slice per GPU). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import everett_amd as ev

@@ -3,8 +3,8 @@
  *
  * This is the drop-in boundary for hz2/everett 0.1.2's simulation paths.  Each
  * entry point names the reference interface it replaces (paths relative to the
- * crate root).  A Rust crate binds these with `extern "C"` (see INTEGRATION.md
- * in the repository); the Python package everett_amd binds them with ctypes.
+ * crate root).  A Rust crate binds these with `extern "C"` declarations of the
+ * same names; the Python package everett_amd binds them with ctypes.
  *
  * Conventions
  *  - Every function returning int returns EV_OK (0) or a negative ev_status.

@@ -2,7 +2,7 @@
 
 The fused path multiplies consecutive one-qubit gates in f64 and evaluates
 gates in a different association order than the reference, so complex128
-results are held to 1e-10 max-abs and complex64 to
+results are held to 1e-10 max-abs (the f64 parity bound) and complex64 to
 1e-5 max-abs with 1 - fidelity <= 1e-6.  At the full 30-qubit size the checks
 are size-independent: a circuit followed by its inverse returns |0..0>, the
 norm is preserved, and QFT of a basis state matches its closed form.
