@@ -336,7 +336,13 @@ __global__ void k_c64_to_c128(const float2 *src, double2 *dst, uint64_t len)
 // targets u * prefix[last] and returns the first index whose running sum
 // (prefix[c-1] + sequential in-chunk sum) exceeds the target.
 
+// Chunks of 2^cl amplitudes, cl = n below 10 qubits, else max(10, n - 16):
+// at most 2^16 chunks, so the sequential prefix over the chunk totals (one
+// dependent f64 add per chunk) stays short at every size, while the chunks
+// stay small enough for the per-chunk sums to run in parallel.
 constexpr int kChunkLog = 10;
+constexpr int kMaxPrefixLog = 16;
+inline int chunk_log(int n) { return n < kChunkLog ? n : std::max(kChunkLog, n - kMaxPrefixLog); }
 
 template <class R> __global__ void k_chunk_totals(const Amp<R> *psi, uint64_t nchunks, uint64_t csize, double *tot)
 {
@@ -349,15 +355,55 @@ template <class R> __global__ void k_chunk_totals(const Amp<R> *psi, uint64_t nc
     }
 }
 
-// Sequential inclusive prefix over the chunk totals (one thread; at most
-// 2^24 chunks), so the CDF has one fixed summation order.
+// The same sums for chunks of at least 64 amplitudes, with coalesced reads:
+// a one-wave workgroup takes 64 chunks; per slab of 64 amplitudes it reads
+// each chunk's slab across the lanes into LDS (|a|^2 per entry), then lane k
+// adds chunk k's 64 weights in index order, so every chunk's sum is formed
+// in the same order as above.
+template <class R>
+__global__ void k_chunk_totals_wide(const Amp<R> *psi, uint64_t nchunks, uint64_t csize, double *tot)
+{
+    __shared__ double w[64][65];  // [offset in slab][chunk of the group], padded against bank conflicts
+    const unsigned lane = threadIdx.x;
+    for (uint64_t c0 = uint64_t(blockIdx.x) * 64; c0 < nchunks; c0 += uint64_t(gridDim.x) * 64) {
+        double s = 0.0;
+        for (uint64_t j0 = 0; j0 < csize; j0 += 64) {
+            for (int k = 0; k < 64; ++k) {
+                const uint64_t c = c0 + k;
+                w[lane][k] = c < nchunks ? nsq(psi[c * csize + j0 + lane]) : 0.0;
+            }
+            __syncthreads();
+            for (int j = 0; j < 64; ++j) s += w[j][lane];
+            __syncthreads();
+        }
+        if (c0 + lane < nchunks) tot[c0 + lane] = s;
+    }
+}
+
+// Sequential inclusive prefix over the chunk totals, so the CDF has one
+// fixed summation order: run = ((t0 + t1) + t2) + ...  One wave reads 64
+// totals at a time (one coalesced load, instead of one dependent load per
+// chunk) and every lane adds them into the same running sum in chunk order
+// (each lane holds the same value; lane k keeps the sum after chunk k).
 __global__ void k_prefix(double *tot, uint64_t nchunks)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (blockIdx.x != 0) return;
+    const unsigned lane = threadIdx.x;  // launched with one 64-lane wave
     double run = 0.0;
-    for (uint64_t c = 0; c < nchunks; ++c) {
-        run += tot[c];
-        tot[c] = run;
+    for (uint64_t base = 0; base < nchunks; base += 64) {
+        const uint64_t c = base + lane;
+        const double v = c < nchunks ? tot[c] : 0.0;
+        const uint64_t vb = __double_as_longlong(v);
+        const uint32_t lo = uint32_t(vb), hi = uint32_t(vb >> 32);
+        double mine = 0.0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {  // past the last chunk this adds +0, which leaves run unchanged
+            const uint64_t b = uint64_t(uint32_t(__builtin_amdgcn_readlane(int(hi), k))) << 32 |
+                               uint32_t(__builtin_amdgcn_readlane(int(lo), k));
+            run += __longlong_as_double(int64_t(b));
+            if (int(lane) == k) mine = run;
+        }
+        if (c < nchunks) tot[c] = mine;
     }
 }
 
@@ -564,21 +610,28 @@ void c64_to_c128(const void *src, void *dst, uint64_t len, hipStream_t st)
 
 size_t sample_scratch_doubles(int n)
 {
-    const int cl = std::min(n, kChunkLog);
+    const int cl = chunk_log(n);
     return size_t(1) << (n - cl);
 }
 
 void sample(const void *psi, Prec p, int n, const double *u, uint64_t shots, double *scratch, uint64_t *out,
             hipStream_t st)
 {
-    const int cl = std::min(n, kChunkLog);
+    const int cl = chunk_log(n);
     const uint64_t csize = uint64_t(1) << cl, nchunks = uint64_t(1) << (n - cl);
-    if (p == C128)
+    if (csize >= 64) {
+        const dim3 g(unsigned(std::min<uint64_t>((nchunks + 63) / 64, kMaxGrid)));
+        if (p == C128)
+            hipLaunchKernelGGL(k_chunk_totals_wide<double>, g, dim3(64), 0, st, A<double>(psi), nchunks, csize, scratch);
+        else
+            hipLaunchKernelGGL(k_chunk_totals_wide<float>, g, dim3(64), 0, st, A<float>(psi), nchunks, csize, scratch);
+    } else if (p == C128) {
         hipLaunchKernelGGL(k_chunk_totals<double>, grid_for(nchunks), dim3(kBlock), 0, st, A<double>(psi), nchunks, csize,
                            scratch);
-    else
+    } else {
         hipLaunchKernelGGL(k_chunk_totals<float>, grid_for(nchunks), dim3(kBlock), 0, st, A<float>(psi), nchunks, csize,
                            scratch);
+    }
     hipLaunchKernelGGL(k_prefix, dim3(1), dim3(64), 0, st, scratch, nchunks);
     if (shots == 0) return;
     if (p == C128)

@@ -778,7 +778,8 @@ size_t ref_gen_dm_random(size_t n, size_t depth, ref_rng *rng, ref_op *out)
 
 /* Definition shared with the product's sampler (no reference counterpart:
  * the reference only measures one qubit at a time, measure.rs:15-51).
- * Chunks of C = 2^min(n,10) amplitudes; total[c] = sequential sum of
+ * Chunks of C = 2^cl amplitudes, cl = n for n < 10, else max(10, n - 16)
+ * (at most 2^16 chunks); total[c] = sequential sum of
  * norm_sqr over the chunk; prefix = sequential inclusive prefix of totals.
  * A shot with uniform u targets u * prefix[last], picks the first chunk with
  * prefix > target (the last chunk if none), then the first index in it whose
@@ -787,7 +788,7 @@ size_t ref_gen_dm_random(size_t n, size_t depth, ref_rng *rng, ref_op *out)
  * chunk's last index. */
 void ref_sample(const rc128 *amps, size_t n, const double *u, size_t shots, uint64_t *out)
 {
-    size_t cl = n < 10 ? n : 10, csize = (size_t)1 << cl, nch = (size_t)1 << (n - cl);
+    size_t cl = n < 10 ? n : (n > 26 ? n - 16 : 10), csize = (size_t)1 << cl, nch = (size_t)1 << (n - cl);
     double *pre = malloc(nch * sizeof(double));
     for (size_t c = 0; c < nch; ++c) {
         double s = 0.0;

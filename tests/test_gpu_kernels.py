@@ -132,8 +132,9 @@ def test_measure_matches_oracle():
             assert r1.state == r2.state
 
 
-@pytest.mark.parametrize("n", [1, 3, 10, 13])
+@pytest.mark.parametrize("n", [1, 3, 10, 13, 27])
 def test_sampling_matches_oracle_exactly(n):
+    # from 27 qubits on the chunks grow past 2^10 amplitudes (at most 2^16 chunks)
     rng = np.random.default_rng(20 + n)
     psi = random_state(rng, n)
     s = ev.State.from_amplitudes(psi)
