@@ -109,9 +109,14 @@ struct ev_server {
     uint64_t next_id = 1;
     ev_server_stats st{};
     ev::BlockCache cache;                              // scheduler thread only
-    ev::fused::BatchEntry *h_ent = nullptr;            // pinned
-    void *d_ent = nullptr;
-    size_t ent_cap = 0;
+    // Two step slots (scheduler thread only): the host fills slot k & 1
+    // while the device may still run the step of the other slot.
+    ev::fused::BatchEntry *h_ent[2] = {nullptr, nullptr};  // pinned
+    void *d_ent[2] = {nullptr, nullptr};
+    size_t ent_cap[2] = {0, 0};
+    hipEvent_t ev_a[2] = {nullptr, nullptr}, ev_b[2] = {nullptr, nullptr};
+    uint64_t step_no = 0;
+    bool inflight = false;                             // step step_no - 1 not yet waited for
     std::thread sched;
 };
 
@@ -203,18 +208,42 @@ void finish(ev_server *s, const std::shared_ptr<Request> &r)
     s->cv_done.notify_all();
 }
 
+// Waits for the step in flight, if any, and adds its device time.
+void drain(ev_server *s)
+{
+    if (!s->inflight) return;
+    const int slot = int((s->step_no - 1) & 1);
+    s->inflight = false;
+    EV_HIP(hipEventSynchronize(s->ev_b[slot]));
+    float ms = 0;
+    EV_HIP(hipEventElapsedTime(&ms, s->ev_a[slot], s->ev_b[slot]));
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.busy_ms += ms;
+    ++s->st.steps;
+}
+
+// One step: ONE launch over the next pass of every active request.  Steps
+// are pipelined two deep: this one is queued before the previous one is
+// waited for, so the host's preparation of a step overlaps the device's
+// run of the one before.
 void step(ev_server *s)
 {
+    const int slot = int(s->step_no & 1);
     const size_t na = s->active.size();
-    if (na > s->ent_cap) {
-        if (s->h_ent) EV_HIP(hipHostFree(s->h_ent));
-        if (s->d_ent) EV_HIP(hipFree(s->d_ent));
-        s->h_ent = nullptr;
-        s->d_ent = nullptr;
+    if (!s->ev_a[slot]) {
+        EV_HIP(hipEventCreate(&s->ev_a[slot]));
+        EV_HIP(hipEventCreate(&s->ev_b[slot]));
+    }
+    if (na > s->ent_cap[slot]) {  // this slot's last step (two ago) has completed
+        if (s->h_ent[slot]) EV_HIP(hipHostFree(s->h_ent[slot]));
+        if (s->d_ent[slot]) EV_HIP(hipFree(s->d_ent[slot]));
+        s->h_ent[slot] = nullptr;
+        s->d_ent[slot] = nullptr;
         const size_t cap = std::max<size_t>(64, 2 * na);
-        EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ent), cap * sizeof(fused::BatchEntry), hipHostMallocDefault));
-        EV_HIP(hipMalloc(&s->d_ent, cap * sizeof(fused::BatchEntry)));
-        s->ent_cap = cap;
+        EV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ent[slot]), cap * sizeof(fused::BatchEntry),
+                             hipHostMallocDefault));
+        EV_HIP(hipMalloc(&s->d_ent[slot], cap * sizeof(fused::BatchEntry)));
+        s->ent_cap[slot] = cap;
     }
     uint64_t tiles = 0;
     int max_t = fused::kMinTile;
@@ -222,7 +251,7 @@ void step(ev_server *s)
         Request &r = *s->active[i];
         unsigned char *d = static_cast<unsigned char *>(r.desc);
         const fused::Pass &ps = r.plan.passes[r.next_pass];
-        fused::BatchEntry &e = s->h_ent[i];
+        fused::BatchEntry &e = s->h_ent[slot][i];
         e.psi = r.psi;
         e.pass = reinterpret_cast<const fused::Pass *>(d + r.pass_off) + r.next_pass;
         e.stages = reinterpret_cast<const fused::Stage *>(d + r.stage_off);
@@ -232,26 +261,15 @@ void step(ev_server *s)
         tiles += uint64_t(1) << (r.n - ps.t);
         max_t = std::max(max_t, int(ps.t));
     }
-    hipEvent_t a, b;
-    EV_HIP(hipEventCreate(&a));
-    EV_HIP(hipEventCreate(&b));
-    hipError_t e = hipMemcpyAsync(s->d_ent, s->h_ent, na * sizeof(fused::BatchEntry), hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess) e = hipEventRecord(a, s->stream);
-    if (e == hipSuccess) {
-        fused::launch_batch(s->c64, static_cast<const fused::BatchEntry *>(s->d_ent), int(na), tiles, max_t, s->stream);
-        e = hipEventRecord(b, s->stream);
-    }
-    if (e == hipSuccess) e = hipEventSynchronize(b);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    EV_HIP(e);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->st.busy_ms += ms;
-        ++s->st.steps;
-    }
+    EV_HIP(hipMemcpyAsync(s->d_ent[slot], s->h_ent[slot], na * sizeof(fused::BatchEntry), hipMemcpyHostToDevice,
+                          s->stream));
+    EV_HIP(hipEventRecord(s->ev_a[slot], s->stream));
+    fused::launch_batch(s->c64, static_cast<const fused::BatchEntry *>(s->d_ent[slot]), int(na), tiles, max_t,
+                        s->stream);
+    EV_HIP(hipEventRecord(s->ev_b[slot], s->stream));
+    drain(s);  // the previous step
+    ++s->step_no;
+    s->inflight = true;
     for (auto &r : s->active) ++r->next_pass;
 }
 
@@ -293,7 +311,11 @@ void scheduler(ev_server *s)
         if (s->active.empty()) continue;
         try {
             step(s);
+            bool completes = false;
+            for (auto &r : s->active) completes = completes || r->next_pass >= r->plan.passes.size();
+            if (completes) drain(s);  // their states are read and released below
         } catch (const Error &e) {
+            s->inflight = false;
             for (auto &r : s->active) {
                 release(s, *r);
                 fail(s, r, e.code);
@@ -351,8 +373,12 @@ void server_destroy(ev_server *s)
     s->cv_work.notify_all();
     if (s->sched.joinable()) s->sched.join();
     (void)hipStreamSynchronize(s->stream);
-    if (s->h_ent) (void)hipHostFree(s->h_ent);
-    if (s->d_ent) (void)hipFree(s->d_ent);
+    for (int k = 0; k < 2; ++k) {
+        if (s->h_ent[k]) (void)hipHostFree(s->h_ent[k]);
+        if (s->d_ent[k]) (void)hipFree(s->d_ent[k]);
+        if (s->ev_a[k]) (void)hipEventDestroy(s->ev_a[k]);
+        if (s->ev_b[k]) (void)hipEventDestroy(s->ev_b[k]);
+    }
     s->cache.release_all();
     (void)hipStreamDestroy(s->stream);
     delete s;
