@@ -145,3 +145,41 @@ def test_tile_cnots_become_exchange_relabellings():
             for o in ops[st["op_begin"]: st["op_end"]]:
                 if decode(o)[0] == CX and int(o["cglob"]):
                     assert not any((int(o["cglob"]) >> q) & 1 for q in tile)
+
+
+def _bank_degree(vecs, bits):
+    """Lanes per LDS bank when lane bit i adds vecs[i] to the slot (mod 2^bits)."""
+    m = (1 << bits) - 1
+    vals = set()
+    for x in range(1 << bits):
+        v = 0
+        for i in range(bits):
+            if (x >> i) & 1:
+                v ^= vecs[i]
+        vals.add(v & m)
+    return (1 << bits) // len(vals)
+
+
+def test_pass_swizzle_keeps_exchanges_conflict_free():
+    # the planner picks each pass's LDS swizzle so the 32 lanes of an 8-byte
+    # exchange access (and the 64 of a 4-byte one) hit distinct banks
+    from tests.fused_emulator import swz
+
+    c = ev.workloads.config1(30, 40)
+    passes, stages, ops, coef, t = get_plan(c, tile_qubits=12, low_qubits=3, max_gates_per_pass=1000)
+    total = free5 = free6 = 0
+    for P in passes:
+        sa, sb = int(P["swz_a"]), int(P["swz_b"])
+        b0, ns = int(P["stage_begin"]), int(P["nstages"])
+        for k in range(ns - 1):
+            w, r = stages[b0 + k], stages[b0 + k + 1]
+            wl = [int(swz(1 << int(w["p"][i]), sa, sb)) for i in range(6)]
+            if int(r["relabel"]):
+                rl = [int(r["rimg"][i]) for i in range(6)]
+            else:
+                rl = [int(swz(1 << int(r["p"][i]), sa, sb)) for i in range(6)]
+            total += 1
+            free5 += _bank_degree(wl, 5) == 1 and _bank_degree(rl, 5) == 1
+            free6 += _bank_degree(wl, 6) == 1 and _bank_degree(rl, 6) == 1
+    assert total > 100
+    assert free5 >= 0.85 * total and free6 >= 0.85 * total, (total, free5, free6)
