@@ -102,7 +102,7 @@ struct Stage {
     int32_t naff;             // relabel: affine terms
     uint32_t abit[kMaxAffine];
     uint32_t aoff[kMaxAffine];
-    int32_t pad2;
+    int32_t gneed;            // an op reads the lane's global index bits, or the stage scatters
 };
 
 struct Pass {

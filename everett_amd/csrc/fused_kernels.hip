@@ -384,16 +384,17 @@ struct Lane {
 // np are masked off (threads beyond a narrower tile alias into it), and the
 // planner zero-fills the unused entries, so the descriptor reads are two
 // wide scalar loads with no wait inside a loop.
-__device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base, uint32_t swz_a, uint32_t swz_b)
+__device__ __forceinline__ Lane lane_of(cptr<Stage> st, uint32_t tid, uint64_t base, uint32_t swz_a, uint32_t swz_b,
+                                       bool with_global = true)
 {
     const uint32_t tm = tid & ((1u << st->np) - 1u);
     uint32_t lb = 0;
     uint64_t g = base;
 #pragma unroll
-    for (int i = 0; i < kMaxTile - kRegBits; ++i) {
-        const uint32_t b = (tm >> i) & 1u;
-        lb |= b << st->p[i];
-        g |= uint64_t(b) << st->gpos[i];
+    for (int i = 0; i < kMaxTile - kRegBits; ++i) lb |= ((tm >> i) & 1u) << st->p[i];
+    if (with_global) {
+#pragma unroll
+        for (int i = 0; i < kMaxTile - kRegBits; ++i) g |= uint64_t((tm >> i) & 1u) << st->gpos[i];
     }
     return {swz(lb, swz_a, swz_b), g};
 }
@@ -529,7 +530,8 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
         run_stage_ops<V>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
             const cptr<Stage> nx = st + 1;
-            const Lane nl = lane_of(nx, tid, tr.base, tr.swz_a, tr.swz_b);
+            // the global index bits only where an op of the stage tests them or the stage scatters
+            const Lane nl = lane_of(nx, tid, tr.base, tr.swz_a, tr.swz_b, nx->gneed != 0);
             uint32_t rb = nl.sb;
             if (nx->relabel) {  // the exchange applies the relabelling CNOTs / SWAPs before the next stage
                 const uint32_t tm = tid & ((1u << nx->np) - 1u);

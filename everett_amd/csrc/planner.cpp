@@ -967,6 +967,11 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef));
             for (const DevOp &d : merge_phases(lowered, P.coef)) P.devops.push_back(d);
             st.op_end = int(P.devops.size());
+            for (int o = st.op_begin; o < st.op_end; ++o) {
+                const DevOp &d = P.devops[size_t(o)];
+                const uint32_t kind = d.code & 0xff;
+                if (d.cglob != 0 || kind == fused::kDiagT || kind == fused::kPhase) st.gneed = 1;
+            }
             P.stages.push_back(st);
         };
 
@@ -1069,6 +1074,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         // stage does the scatter.
         if ((lastS & lowc) || in_pending) emit_stage(pad_registers(0, t, lowc, true), {}, in_pending ? &a_in : nullptr);
         ps.nstages = int(P.stages.size()) - ps.stage_begin;
+        P.stages.back().gneed = 1;  // the last stage scatters to global addresses
         pass_swizzle(ps, P.stages.data() + ps.stage_begin);
         P.passes.push_back(ps);
     }

@@ -22,7 +22,7 @@ PASS_DTYPE = np.dtype([("t", "<i4"), ("nstages", "<i4"), ("stage_begin", "<i4"),
 STAGE_DTYPE = np.dtype([("op_begin", "<i4"), ("op_end", "<i4"), ("np", "<i4"), ("relabel", "<i4"),
                         ("roff", "<u4", (16,)), ("goff", "<u8", (16,)), ("p", "<i4", (12,)), ("gpos", "<i4", (12,)),
                         ("rimg", "<u4", (12,)), ("rroff", "<u4", (16,)), ("naff", "<i4"), ("abit", "<u4", (8,)),
-                        ("aoff", "<u4", (8,)), ("pad2", "<i4")])
+                        ("aoff", "<u4", (8,)), ("gneed", "<i4")])
 OP_DTYPE = np.dtype([("code", "<u4"), ("creg", "<u4"), ("cglob", "<u8"), ("coef", "<u4"), ("tq", "<u4"),
                      ("pad", "<u4", (2,)), ("m", "<c16", (4,))])
 
@@ -132,9 +132,13 @@ def check_plan(n, passes, stages, ops, coef, coalesce=3):
                 gj = int(st["goff"][1 << j])
                 assert gj and gj & (gj - 1) == 0
                 regq.add(gj.bit_length() - 1)
+            if s == P["nstages"] - 1:
+                assert int(st["gneed"]) == 1, "the scattering stage needs the lanes' global index bits"
             for o in ops[st["op_begin"]: st["op_end"]]:
                 k, b0, b1, _ = decode(o)
                 assert k <= PHASE and o["creg"] < 16
+                if int(o["cglob"]) or k in (DIAGT, PHASE):
+                    assert int(st["gneed"]) == 1, "an op that tests global index bits needs them computed"
                 if k == PHASE:
                     assert b0 <= PHASE_ALL and 2 <= b1 <= MAX_PHASE_TERMS and o["cglob"] == 0
                     assert b0 == PHASE_ALL or not (int(o["creg"]) >> b0) & 1
