@@ -109,19 +109,35 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
         graph = nullptr;
     };
     try {
-        // ---- validate and size
+        // ---- validate (on host threads; the first failing circuit in index
+        // order is the one reported) and size
         b->n.resize(count);
         b->aoff.resize(count + 1);
         b->gates.resize(count);
+        {
+            std::vector<std::exception_ptr> verr(count);
+            parallel_for(0, count, [&](size_t i) {
+                try {
+                    const Circuit &c = *cs[i];
+                    c.validate();
+                    const uint64_t n = c.num_qubits();
+                    EV_REQUIRE(n >= uint64_t(fused::kMinTile) && n <= 30, EV_E_LIMIT,
+                               "batched circuits need 8..30 qubits");
+                    for (const Op &op : c.ops())
+                        EV_REQUIRE(op.kind == OpKind::Apply1 || op.kind == OpKind::Apply2 ||
+                                       op.kind == OpKind::Controlled,
+                                   EV_E_ARG, "batched circuits must be measurement-free");
+                } catch (...) {
+                    verr[i] = std::current_exception();
+                }
+            });
+            for (auto &e : verr)
+                if (e) std::rethrow_exception(e);
+        }
         uint64_t amps = 0;
         for (size_t i = 0; i < count; ++i) {
             const Circuit &c = *cs[i];
-            c.validate();
             const uint64_t n = c.num_qubits();
-            EV_REQUIRE(n >= uint64_t(fused::kMinTile) && n <= 30, EV_E_LIMIT, "batched circuits need 8..30 qubits");
-            for (const Op &op : c.ops())
-                EV_REQUIRE(op.kind == OpKind::Apply1 || op.kind == OpKind::Apply2 || op.kind == OpKind::Controlled,
-                           EV_E_ARG, "batched circuits must be measurement-free");
             b->n[i] = int(n);
             b->gates[i] = c.size();
             b->aoff[i] = amps;
