@@ -47,6 +47,12 @@ void scale(void *psi, Prec p, int n, double s, hipStream_t st);
 // Streaming copy of `bytes` (a multiple of 32), the HBM bandwidth reference.
 void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 
+// SWAPs of the qubit pairs (a[k], b[k]) (pairwise disjoint) on a 2^n state,
+// in place in one pass: amplitude x trades places with pi(x), pi exchanging
+// the bits of every pair.
+constexpr int kMaxSwapPairs = 17;
+void swap_bits(void *psi, Prec p, int n, const int *a, const int *b, int npairs, hipStream_t st);
+
 // Precision conversion between a c128 staging buffer and c64 storage.
 void c128_to_c64(const void *src, void *dst, uint64_t len, hipStream_t st);
 void c64_to_c128(const void *src, void *dst, uint64_t len, hipStream_t st);

@@ -1178,15 +1178,59 @@ double now_ms()
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A segment that ends in pairwise-disjoint SWAPs (a QFT's bit reversal) on
+// qubits spread wider than one tile would spend whole passes moving
+// amplitudes.  Those SWAPs whose qubits are all above the coalesced low ones
+// run instead as one in-place bit-permutation pass after the plan (they
+// commute with each other, and nothing follows them in the segment); the
+// others stay in the plan.  Returns the peeled pairs (empty if not worth it).
+std::vector<std::pair<int, int>> peel_swap_tail(Lowering &low, const FusedConfig &cfg, int n)
+{
+    std::vector<std::pair<int, int>> tail;
+    std::vector<GOp> kept;
+    uint64_t used = 0;
+    size_t keep = low.ops.size();
+    while (keep > 0) {
+        const GOp &g = low.ops[keep - 1];
+        if (g.kind != G_SWAP || g.cmask) break;
+        const uint64_t m = (uint64_t(1) << g.q0) | (uint64_t(1) << g.q1);
+        if (m & used) break;
+        used |= m;
+        if (std::min(g.q0, g.q1) >= cfg.coalesce) tail.emplace_back(g.q0, g.q1);
+        else kept.push_back(g);
+        --keep;
+    }
+    const bool worth = n > cfg.t && int(2 * tail.size()) > cfg.t - cfg.low && int(tail.size()) <= kern::kMaxSwapPairs;
+    if (!worth) return {};
+    low.ops.resize(keep);
+    for (auto it = kept.rbegin(); it != kept.rend(); ++it) low.ops.push_back(*it);
+    return tail;
+}
+
 void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st, Profiler &prof)
 {
     low.flush_all();
     if (low.ops.empty()) return;
     const double t0 = now_ms();
+    const std::vector<std::pair<int, int>> tail = peel_swap_tail(low, cfg, s->n);
     const float budget = (cfg.t >= s->n) ? 1e30f : cfg.budget;  // one tile holds the state: one pass
-    const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce);
-    st.plan_ms += now_ms() - t0;
-    run_plan(s, P, st, prof);
+    if (!low.ops.empty()) {
+        const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce);
+        st.plan_ms += now_ms() - t0;
+        run_plan(s, P, st, prof);
+    }
+    if (!tail.empty()) {
+        std::vector<int> a, b;
+        for (const auto &pr : tail) {
+            a.push_back(pr.first);
+            b.push_back(pr.second);
+        }
+        prof.begin();
+        kern::swap_bits(s->psi, s->prec, s->n, a.data(), b.data(), int(tail.size()), s->stream);
+        prof.end("swap_bits", full_pass_bytes(s->n, kern::amp_bytes(s->prec)));
+        ++st.passes;
+        ++st.launches;
+    }
     low.ops.clear();
     low.pool.clear();
 }

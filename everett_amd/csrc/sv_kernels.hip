@@ -192,6 +192,33 @@ __global__ void k_apply2_dense(Amp<R> *psi, uint64_t ngroups, int lo, int hi, ui
     }
 }
 
+struct SwapPairs {
+    int count;
+    int a[kMaxSwapPairs], b[kMaxSwapPairs];
+};
+
+// kernel.rs:277-289 for several disjoint pairs at once: pi(x) exchanges the
+// bits a[k] and b[k] of x for every pair; pi is an involution, so the
+// amplitudes at x and pi(x) trade places (the thread with x < pi(x) moves
+// both; fixed points stay).  A pair's bits are never the lowest ones the
+// caller keeps contiguous, so runs of consecutive x map to runs of pi(x).
+template <class R> __global__ void k_swap_bits(Amp<R> *psi, uint64_t len, SwapPairs sp)
+{
+    GRID_STRIDE(x, len)
+    {
+        uint64_t y = x;
+        for (int k = 0; k < sp.count; ++k) {
+            const uint64_t d = ((x >> sp.a[k]) ^ (x >> sp.b[k])) & 1;
+            y ^= (d << sp.a[k]) | (d << sp.b[k]);
+        }
+        if (x < y) {
+            const Amp<R> u = psi[x], v = psi[y];
+            psi[x] = v;
+            psi[y] = u;
+        }
+    }
+}
+
 struct CtrlArgs {
     int npos;            // controls + target, ascending
     int pos[EV_MAX_CONTROLS + 1];
@@ -511,6 +538,20 @@ __global__ void k_copy(const double4 *__restrict__ src, double4 *__restrict__ ds
 }
 
 }  // namespace
+
+void swap_bits(void *psi, Prec p, int n, const int *a, const int *b, int npairs, hipStream_t st)
+{
+    EV_REQUIRE(npairs >= 0 && npairs <= kMaxSwapPairs, EV_E_ARG, "too many swap pairs");
+    SwapPairs sp{};
+    sp.count = npairs;
+    for (int k = 0; k < npairs; ++k) {
+        sp.a[k] = a[k];
+        sp.b[k] = b[k];
+    }
+    const uint64_t len = uint64_t(1) << n;
+    if (p == C128) hipLaunchKernelGGL(k_swap_bits<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, sp);
+    else hipLaunchKernelGGL(k_swap_bits<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, sp);
+}
 
 void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st)
 {

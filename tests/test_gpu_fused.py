@@ -197,3 +197,32 @@ def test_run_stats_count_algorithmic_pass_flops():
     c.cnot(2, 6)
     ev.StateVectorBackend.run(c)
     assert ev.last_run_stats().pass_flops == 0
+
+
+def test_trailing_swap_network_runs_as_one_permutation_pass():
+    # a segment ending in disjoint SWAPs spread wider than a tile: those above
+    # the coalesced low qubits run as one in-place bit-permutation pass, the
+    # ones on low qubits stay in the plan; a measurement ends the first segment
+    n = 18
+    c = ev.Circuit.with_classical(n, 1)
+    c.compose(ev.workloads.brickwall(n, 3, ev.Rng.seed_from_u64(21)))
+    for i in range(n // 2):
+        c.swap(i, n - 1 - i)
+    c.measure(5, 0)
+    c.compose(ev.workloads.brickwall(n, 2, ev.Rng.seed_from_u64(22)))
+    for i in range(2, 8):
+        c.swap(i, i + 9)
+    for seed in (0, 3):
+        ex = ev.StateVectorBackend.run_seeded(c, seed, tile_qubits=12)
+        want, bits = oracle.sv_run(c.ops(), n, seed, num_classical=1)
+        assert ex.classical() == bits
+        assert np.max(np.abs(ex.state().amplitudes() - want)) < TOL128
+    # measurement-free, profiled: the permutation pass ran
+    c2 = ev.workloads.brickwall(n, 3, ev.Rng.seed_from_u64(23))
+    for i in range(n // 2):
+        c2.swap(i, n - 1 - i)
+    st = ev.State.zero(n, "c128")
+    ev.run_into(st, c2, 0, tile_qubits=12, profile=True)
+    assert "swap_bits" in {k["name"] for k in ev.last_kernel_stats()}
+    want, _ = oracle.sv_run(c2.ops(), n)
+    assert np.max(np.abs(st.amplitudes() - want)) < TOL128
