@@ -1,7 +1,7 @@
 """States past 2^32 amplitudes (64 GiB each): the 64-bit index paths of the
-fused passes, the plain gate kernels and the range copies, checked through
-properties that hold at any size (a round trip back to |0>, a product state's
-closed form)."""
+fused passes, the plain gate kernels, the density-matrix channels and
+reductions and the range copies, checked through properties that hold at any
+size (a round trip back to |0>, a product state's closed form)."""
 import gc
 import math
 
@@ -70,4 +70,32 @@ def test_product_state_closed_form_at_33_qubits_c64(plain):
         assert np.max(np.abs(got - want)) < 1e-5 * amp, (plain, lo)
     assert abs(s.norm_sqr() - 1.0) < 1e-4
     del s
+    gc.collect()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_depolarized_product_state_at_16_qubit_density_matrix(plain):
+    # rho of 16 qubits is a 32-qubit vector (64 GiB).  H then a phase on every
+    # qubit, a depolarizing channel (probability p) after each gate: each
+    # qubit's Bloch vector is s^2 (cos phi, sin phi, 0) with s = 1 - 4p/3, so
+    # the purity is ((1 + s^4) / 2)^n and every basis state has probability 2^-n
+    n, p = 16, 0.03
+    phis = [0.1 + 0.37 * q for q in range(n)]
+    c = ev.Circuit(n)
+    for q in range(n):
+        c.h(q)
+    for q in range(n):
+        c.phase(q, phis[q])
+    rho = ev.DensityMatrixBackend.run_with_noise_seeded(
+        c, ev.NoiseModel.uniform_depolarizing(p), 0, plain=plain).density_matrix()
+    s2 = (1 - 4 * p / 3) ** 2
+    assert abs(rho.trace() - 1.0) < 1e-10
+    assert abs(rho.purity() - ((1 + s2 * s2) / 2) ** n) < 1e-10
+    for q in (0, 9, 15):
+        assert abs(rho.expectation_pauli(q, "X") - s2 * math.cos(phis[q])) < 1e-10, q
+        assert abs(rho.expectation_pauli(q, "Y") - s2 * math.sin(phis[q])) < 1e-10, q
+        assert abs(rho.expectation_pauli(q, "Z")) < 1e-10, q
+    for basis in (0, 0xBEEF, 0xFFFF):
+        assert abs(rho.probability(basis) - 2.0 ** -n) < 1e-12, basis
+    del rho
     gc.collect()
