@@ -95,6 +95,9 @@ def lib():
             "ref_gen_brickwall": (SZ, [SZ, SZ, R, P]),
             "ref_gen_dm_random": (SZ, [SZ, SZ, R, P]),
             "ref_sample": (None, [P, SZ, P, SZ, P]),
+            "ref_gate_compose": (None, [P, P, P]),
+            "ref_gate_adjoint": (None, [P, P]),
+            "ref_gate_controlled": (None, [P, P]),
         }
         for g in ("h", "x", "y", "z", "s", "t", "cnot", "cz", "swap"):
             protos[f"ref_gate_{g}"] = (None, [P])
@@ -131,6 +134,30 @@ def gate2(name: str) -> np.ndarray:
     m = np.zeros(16, dtype=np.complex128)
     getattr(lib(), f"ref_gate_{name}")(_p(m))
     return m
+
+
+def gate_compose(a, b) -> np.ndarray:
+    """Gate1::compose (src/gate.rs:153-162): a * b, the gate that applies b then a."""
+    aa, bb = _mat(a, 4), _mat(b, 4)
+    out = np.zeros(4, dtype=np.complex128)
+    lib().ref_gate_compose(_p(aa), _p(bb), _p(out))
+    return out
+
+
+def gate_adjoint(a) -> np.ndarray:
+    """Gate1::adjoint (src/gate.rs:142-150)."""
+    aa = _mat(a, 4)
+    out = np.zeros(4, dtype=np.complex128)
+    lib().ref_gate_adjoint(_p(aa), _p(out))
+    return out
+
+
+def gate_controlled(u) -> np.ndarray:
+    """Gate2::controlled (src/gate.rs:240-247): diag(I, u) in the |q_a q_b> basis."""
+    uu = _mat(u, 4)
+    out = np.zeros(16, dtype=np.complex128)
+    lib().ref_gate_controlled(_p(uu), _p(out))
+    return out
 
 
 def set_threads(n: int) -> None:
@@ -203,6 +230,11 @@ def apply_controlled_1q(amps: np.ndarray, controls, target: int, m) -> None:
 
 def state_norm_sqr(amps: np.ndarray) -> float:
     return float(lib().ref_state_norm_sqr(_p(amps), amps.size))
+
+
+def state_normalize(amps: np.ndarray) -> None:
+    """State::normalize (src/state.rs:103-111), in place."""
+    lib().ref_state_normalize(_p(amps), amps.size)
 
 
 def state_overlap(a: np.ndarray, other: np.ndarray) -> complex:

@@ -118,6 +118,13 @@ void ref_gate_compose(const rc128 a[4], const rc128 b[4], rc128 out[4])
     set4(out, r0, r1, r2, r3);
 }
 
+/* gate.rs:142-150: conjugate transpose */
+void ref_gate_adjoint(const rc128 a[4], rc128 out[4])
+{
+    rc128 r0 = C(a[0].re, -a[0].im), r1 = C(a[2].re, -a[2].im), r2 = C(a[1].re, -a[1].im), r3 = C(a[3].re, -a[3].im);
+    set4(out, r0, r1, r2, r3);
+}
+
 static void eye4(rc128 m[16]) { memset(m, 0, 16 * sizeof(rc128)); for (int i = 0; i < 4; ++i) m[i * 5] = C(1, 0); }
 /* gate.rs:190-198 */
 void ref_gate_cnot(rc128 m[16]) { eye4(m); m[10] = C(0, 0); m[11] = C(1, 0); m[14] = C(1, 0); m[15] = C(0, 0); }
@@ -125,6 +132,12 @@ void ref_gate_cnot(rc128 m[16]) { eye4(m); m[10] = C(0, 0); m[11] = C(1, 0); m[1
 void ref_gate_cz(rc128 m[16]) { eye4(m); m[15] = C(-1, 0); }
 /* gate.rs:225-232 */
 void ref_gate_swap(rc128 m[16]) { eye4(m); m[5] = C(0, 0); m[6] = C(1, 0); m[9] = C(1, 0); m[10] = C(0, 0); }
+/* gate.rs:240-247: diag(I_2, u) in the |q_a q_b> basis */
+void ref_gate_controlled(const rc128 u[4], rc128 m[16])
+{
+    eye4(m);
+    m[10] = u[0]; m[11] = u[1]; m[14] = u[2]; m[15] = u[3];
+}
 
 /* --------------------------------------------------------------- kernel.rs */
 

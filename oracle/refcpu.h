@@ -72,9 +72,11 @@ void ref_gate_ry(double theta, rc128 m[4]);
 void ref_gate_rz(double theta, rc128 m[4]);
 void ref_gate_phase(double lambda, rc128 m[4]);
 void ref_gate_compose(const rc128 a[4], const rc128 b[4], rc128 out[4]); /* gate.rs:153-162 */
+void ref_gate_adjoint(const rc128 a[4], rc128 out[4]);                   /* gate.rs:142-150 */
 void ref_gate_cnot(rc128 m[16]);
 void ref_gate_cz(rc128 m[16]);
 void ref_gate_swap(rc128 m[16]);
+void ref_gate_controlled(const rc128 u[4], rc128 m[16]);                /* gate.rs:240-247 */
 
 /* ---- kernel.rs ---- */
 void ref_index_pair(size_t k, size_t i, size_t out[2]);                 /* kernel.rs:37-44 */

@@ -73,6 +73,150 @@ def test_next_f64_is_top_53_bits():
         assert r1.next_f64() == (r2.next_u64() >> 11) * 2.0**-53
 
 
+# ------------------------------------------------------------------- gates
+#
+# Textbook definitions (src/gate.rs:62-136 doc comments: R_a(t) = exp(-i t a/2)
+# for the Pauli a) computed independently with scipy's matrix exponential,
+# the reference's own identity tests (src/gate.rs:297-360), and compose
+# against numpy's matrix product.  A sign slip in a rotation, a swapped
+# sin/cos, a transposed constructor or a reversed compose fails one of them.
+
+PAULI = {
+    "x": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+ANGLES = [0.0, 1e-9, 0.3, -0.7, math.pi / 2, math.pi, 2.1, -math.pi, 3 * math.pi / 2, 2 * math.pi, 5.5, -11.0]
+
+
+def _m2(v):
+    return np.asarray(v, dtype=np.complex128).reshape(2, 2)
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_rotations_match_matrix_exponential(axis):
+    from scipy.linalg import expm
+
+    for th in ANGLES + list(np.linspace(-4 * math.pi, 4 * math.pi, 37)):
+        want = expm(-0.5j * th * PAULI[axis])
+        got = _m2(oracle.gate1("r" + axis, th))
+        assert np.max(np.abs(got - want)) < 2e-15, (axis, th)
+
+
+def test_fixed_gates_match_textbook():
+    s = math.sqrt(0.5)  # correctly rounded 1/sqrt(2), the value of FRAC_1_SQRT_2 (gate.rs:81-85)
+    assert np.array_equal(_m2(oracle.gate1("x")), PAULI["x"])
+    assert np.array_equal(_m2(oracle.gate1("y")), PAULI["y"])
+    assert np.array_equal(_m2(oracle.gate1("z")), PAULI["z"])
+    assert np.array_equal(_m2(oracle.gate1("h")), np.array([[s, s], [s, -s]], dtype=np.complex128))
+    assert np.array_equal(_m2(oracle.gate1("s")), np.diag([1, 1j]))
+    assert np.max(np.abs(_m2(oracle.gate1("t")) - np.diag([1, np.exp(1j * math.pi / 4)]))) < 2e-16
+    for lam in ANGLES:
+        assert np.max(np.abs(_m2(oracle.gate1("phase", lam)) - np.diag([1, np.exp(1j * lam)]))) < 2e-16
+
+
+def test_two_qubit_gates_match_textbook():
+    # rows/columns |q_a q_b> = |00>, |01>, |10>, |11> (src/gate.rs:186-232)
+    perm = lambda p: np.eye(4, dtype=np.complex128)[p]  # noqa: E731  row r has its 1 in column p[r]
+    assert np.array_equal(oracle.gate2("cnot").reshape(4, 4), perm([0, 1, 3, 2]))
+    assert np.array_equal(oracle.gate2("swap").reshape(4, 4), perm([0, 2, 1, 3]))
+    assert np.array_equal(oracle.gate2("cz").reshape(4, 4), np.diag([1, 1, 1, -1]).astype(np.complex128))
+    # CNOT acting on a product state |a>|b> flips b exactly when a is set
+    for a in range(2):
+        for b in range(2):
+            e = np.zeros(4)
+            e[2 * a + b] = 1
+            out = oracle.gate2("cnot").reshape(4, 4) @ e
+            assert out[2 * a + (b ^ a)] == 1
+
+
+def test_compose_is_matrix_product_in_order():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        a = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        b = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        got = _m2(oracle.gate_compose(a, b))
+        assert np.max(np.abs(got - a @ b)) < 1e-14
+        assert np.max(np.abs(a @ b - b @ a)) > 1e-3  # the order is observable
+        assert np.array_equal(_m2(oracle.gate_adjoint(a)), a.conj().T)
+
+
+def _is_identity(m, eps=1e-12):  # gate.rs:166-170
+    return np.max(np.abs(_m2(m) - np.eye(2))) <= eps
+
+
+def _is_unitary(m, eps=1e-12):  # gate.rs:174-177
+    return _is_identity(oracle.gate_compose(oracle.gate_adjoint(m), m), eps)
+
+
+def test_reference_gate_identities():  # src/gate.rs:297-360, restated on the oracle
+    g = {k: oracle.gate1(k) for k in ("x", "y", "z", "h", "s", "t")}
+    for name, m in g.items():  # paulis_are_unitary
+        assert _is_unitary(m), name
+    for th in ANGLES:
+        for r in ("rx", "ry", "rz", "phase"):
+            assert _is_unitary(oracle.gate1(r, th)), (r, th)
+    assert _is_identity(oracle.gate_compose(g["x"], g["x"]))  # pauli_x_is_its_own_inverse
+    zd = oracle.gate_adjoint(g["z"])
+    s2 = oracle.gate_compose(g["s"], g["s"])
+    assert _is_identity(oracle.gate_compose(s2, zd))  # s_squared_is_z
+    t4 = oracle.gate_compose(oracle.gate_compose(oracle.gate_compose(g["t"], g["t"]), g["t"]), g["t"])
+    assert _is_identity(oracle.gate_compose(t4, zd))  # t_to_the_fourth_is_z
+    assert not _is_identity(oracle.gate_compose(oracle.gate_compose(g["t"], g["t"]), zd))  # T^2 = S, not Z
+    rz = oracle.gate1("rz", math.pi)
+    adjusted = np.array([oracle.cmul(complex(v), 1j) for v in rz])  # rz_pi_matches_z_up_to_global_phase
+    assert _is_identity(oracle.gate_compose(adjusted, zd))
+    for name in ("cnot", "cz", "swap"):  # two_qubit_gates_are_unitary
+        u = oracle.gate2(name).reshape(4, 4)
+        assert np.max(np.abs(u.conj().T @ u - np.eye(4))) <= 1e-12
+    cx = oracle.gate_controlled(g["x"])  # controlled_x_is_cnot
+    assert np.array_equal(cx, oracle.gate2("cnot"))
+    assert np.array_equal(oracle.gate_controlled(g["z"]), oracle.gate2("cz"))  # controlled_z_matches_cz
+    u = random_unitary(np.random.default_rng(3), 2)
+    want = np.eye(4, dtype=np.complex128)
+    want[2:, 2:] = u
+    assert np.array_equal(oracle.gate_controlled(u).reshape(4, 4), want)
+
+
+# ------------------------------------------------------------------- state
+
+def test_overlap_is_conjugate_linear_in_other():  # state.rs:130-138: <other|self> = sum conj(other_j) self_j
+    rng = np.random.default_rng(11)
+    for n in (1, 3, 6):
+        a = random_state(rng, n) * 1.7
+        b = random_state(rng, n) * (0.4 - 0.9j)
+        got = oracle.state_overlap(a, b)
+        assert abs(got - np.vdot(b, a)) < 1e-13
+        assert abs(np.vdot(b, a) - np.vdot(a, b)) > 1e-3  # the conjugated operand is observable
+
+
+def test_fidelity_ignores_global_phase():  # state.rs:208-217
+    a = np.zeros(4, dtype=np.complex128)
+    a[0] = 1
+    b = a * np.exp(0.7j)
+    assert abs(abs(oracle.state_overlap(a, b)) ** 2 - 1.0) < 1e-12
+    rng = np.random.default_rng(4)
+    psi = random_state(rng, 5)
+    assert abs(abs(oracle.state_overlap(psi, psi * np.exp(-2.3j))) ** 2 - 1.0) < 1e-12
+
+
+def test_normalize_matches_numpy():  # state.rs:103-111 and normalize_fixes_drift (state.rs:227-233)
+    s = np.array([2.0, 2.0j], dtype=np.complex128)
+    oracle.state_normalize(s)
+    assert abs(oracle.state_norm_sqr(s) - 1.0) < 1e-12
+    assert np.max(np.abs(s - np.array([1, 1j]) / math.sqrt(2))) < 1e-15
+    rng = np.random.default_rng(8)
+    for n in (1, 4, 9):
+        v = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)) * 3.1
+        want = v / np.linalg.norm(v)
+        assert abs(oracle.state_norm_sqr(v) - np.vdot(v, v).real) < 1e-10
+        oracle.state_normalize(v)
+        assert np.max(np.abs(v - want)) < 1e-15
+    z = np.zeros(8, dtype=np.complex128)  # norm 0: left alone (the `if norm > 0.0` guard)
+    oracle.state_normalize(z)
+    assert not np.any(z)
+
+
 # ---------------------------------------------------------------- index maps
 
 @pytest.mark.parametrize("q", range(1, 9))
@@ -264,6 +408,48 @@ def test_brickwall_structure():
     # layer 1 CNOTs start at qubit 1
     layer1 = ops[20 * 3 + 10 + 20 * 3: 20 * 3 + 10 + 20 * 3 + 9]
     assert list(layer1["target"]) == list(range(1, 19, 2))
+
+
+def test_dm_random_structure():
+    """gen_dm_random against the draw order written for it (Fisher-Yates
+    i = n-1..1 with j = next_u64() % (i+1); per pair r = next_u64() % 6, r == 5 a
+    CNOT, else kind r then kind next_u64() % 5; a leftover qubit draws one kind;
+    Rx/Rz draw TAU * next_f64()), re-derived here from the stream."""
+    n, depth = 7, 4
+    ops = oracle.gen_dm_random(n, depth, oracle.Rng(13))
+    r = oracle.Rng(13)
+    want = []
+
+    def one(kind, q):
+        if kind in (3, 4):
+            th = 2 * math.pi * r.next_f64()
+            want.append((1, q, oracle.gate1("rx" if kind == 3 else "rz", th)))
+        else:
+            want.append((1, q, oracle.gate1(("h", "s", "t")[kind])))
+
+    for _ in range(depth):
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = r.next_u64() % (i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        p = 0
+        while p + 1 < n:
+            k = r.next_u64() % 6
+            if k == 5:
+                want.append((2, perm[p], perm[p + 1]))
+            else:
+                one(k, perm[p])
+                one(r.next_u64() % 5, perm[p + 1])
+            p += 2
+        if p < n:
+            one(r.next_u64() % 5, perm[p])
+    assert len(ops) == len(want)
+    for o, w in zip(ops, want):
+        assert int(o["kind"]) == w[0] and int(o["target"]) == w[1]
+        if w[0] == 1:
+            assert np.array_equal(o["m"][:4], w[2])
+        else:
+            assert int(o["b"]) == w[2] and np.array_equal(o["m"], oracle.gate2("cnot"))
 
 
 def test_if_classic_and_measure_drive():  # backend/mod.rs:51-72
