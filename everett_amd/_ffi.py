@@ -12,7 +12,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libeverett.so"
 
-MAX_CONTROLS = 16
+MAX_CONTROLS = 16  # controls carried per ev_op record (an op's controls continue in OP_CONTROLS records)
 
 # ev_op (include/everett.h) as a numpy structured dtype; 408 bytes.
 OP_DTYPE = np.dtype(
@@ -28,7 +28,41 @@ OP_DTYPE = np.dtype(
 )
 assert OP_DTYPE.itemsize == 408
 
-OP_APPLY1, OP_APPLY2, OP_CONTROLLED, OP_MEASURE, OP_IF = 1, 2, 3, 4, 5
+OP_APPLY1, OP_APPLY2, OP_CONTROLLED, OP_MEASURE, OP_IF, OP_CONTROLS = 1, 2, 3, 4, 5, 6
+
+
+def controlled_records(target: int, m, controls, dtype=None) -> np.ndarray:
+    """An Op::Controlled as ev_op records: the head carries the total control
+    count and the first MAX_CONTROLS controls, OP_CONTROLS records the rest."""
+    controls = [int(c) for c in controls]
+    extra = max(0, len(controls) - MAX_CONTROLS)
+    out = np.zeros(1 + (extra + MAX_CONTROLS - 1) // MAX_CONTROLS, dtype=dtype or OP_DTYPE)
+    out[0]["kind"] = OP_CONTROLLED
+    out[0]["target"] = target
+    mm = np.asarray(m, dtype=np.complex128).reshape(-1)
+    out[0]["m"][: mm.size] = mm
+    out[0]["num_controls"] = len(controls)
+    head = controls[:MAX_CONTROLS]
+    out[0]["controls"][: len(head)] = head
+    for r in range(1, len(out)):
+        part = controls[MAX_CONTROLS * r: MAX_CONTROLS * (r + 1)]
+        out[r]["kind"] = OP_CONTROLS
+        out[r]["num_controls"] = len(part)
+        out[r]["controls"][: len(part)] = part
+    return out
+
+
+def decode_controls(ops: np.ndarray, i: int):
+    """The controls of the controlled op whose head record is ops[i], and the
+    index of the record after it."""
+    total = int(ops[i]["num_controls"])
+    cs = [int(x) for x in ops[i]["controls"][: min(total, MAX_CONTROLS)]]
+    j = i + 1
+    while len(cs) < total:
+        k = int(ops[j]["num_controls"])
+        cs += [int(x) for x in ops[j]["controls"][:k]]
+        j += 1
+    return cs, j
 
 EV_C128, EV_C64 = 0, 1
 

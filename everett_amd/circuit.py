@@ -55,7 +55,8 @@ class Circuit:
         return int(_ffi.lib().ev_circuit_num_classical(self._h))
 
     def __len__(self) -> int:
-        """Number of op records (an IfClassic is its If record plus the guarded op)."""
+        """Number of op records (an IfClassic is its If record plus the guarded op; a
+        controlled op with more than 16 controls takes continuation records)."""
         return int(_ffi.lib().ev_circuit_len(self._h))
 
     def is_empty(self) -> bool:
@@ -75,18 +76,13 @@ class Circuit:
         check(_ffi.lib().ev_circuit_append(self._h, _ffi.ptr(ops), len(ops)))
         return self
 
-    def _push(self, kind, target=0, b=0, m=None, controls=()):
+    def _push(self, kind, target=0, b=0, m=None):
         rec = np.zeros(1, dtype=_ffi.OP_DTYPE)
         rec["kind"] = kind
         rec["target"] = target
         rec["b"] = b
         if m is not None:
             rec["m"][0, : len(m)] = m
-        if controls:
-            if len(controls) > _ffi.MAX_CONTROLS:
-                raise ValueError(f"at most {_ffi.MAX_CONTROLS} controls per op")
-            rec["num_controls"] = len(controls)
-            rec["controls"][0, : len(controls)] = controls
         return self.append_ops(rec)
 
     # --- generic builders (circuit.rs:91-127) ----------------------------
@@ -97,7 +93,8 @@ class Circuit:
         return self._push(_ffi.OP_APPLY2, a, b, gate.m)
 
     def controlled(self, controls, gate: Gate1, target: int) -> "Circuit":
-        return self._push(_ffi.OP_CONTROLLED, target, 0, gate.m, tuple(int(c) for c in controls))
+        """Circuit::controlled (src/circuit.rs:104-116): any number of controls."""
+        return self.append_ops(_ffi.controlled_records(target, gate.m, controls))
 
     def compose(self, sub: "Circuit") -> "Circuit":
         check(_ffi.lib().ev_circuit_compose(self._h, sub._h))
@@ -160,7 +157,8 @@ class Circuit:
     def num_ops(self) -> int:
         """Number of ops as the reference counts them (Circuit::len, src/circuit.rs:68-71):
         an IfClassic and the op it guards are one op."""
-        return int(np.count_nonzero(self.ops()["kind"] != _ffi.OP_IF))
+        kinds = self.ops()["kind"]
+        return int(np.count_nonzero((kinds != _ffi.OP_IF) & (kinds != _ffi.OP_CONTROLS)))
 
     # --- OpenQASM 3 -------------------------------------------------------
     @staticmethod

@@ -87,7 +87,6 @@ void check_q(uint64_t q, int n)
 
 void check_controls(const uint64_t *controls, size_t nc, uint64_t target, int n)
 {
-    EV_REQUIRE(nc <= EV_MAX_CONTROLS, EV_E_ARG, "more than EV_MAX_CONTROLS controls");
     EV_REQUIRE(nc == 0 || controls != nullptr, EV_E_ARG, "controls is null");
     check_q(target, n);
     for (size_t i = 0; i < nc; ++i) {
@@ -122,7 +121,7 @@ int ev_circuit_append(ev_circuit *c, const ev_op *ops, size_t count)
     return guard([&] {
         NONNULL(c);
         EV_REQUIRE(count == 0 || ops != nullptr, EV_E_ARG, "ops is null");
-        for (size_t i = 0; i < count; ++i) c->c.append(ops[i]);
+        for (size_t i = 0; i < count;) i = c->c.append(ops, count, i);
     });
 }
 
@@ -136,7 +135,7 @@ int ev_circuit_compose(ev_circuit *dst, const ev_circuit *src)
     });
 }
 
-size_t ev_circuit_len(const ev_circuit *c) { return c ? c->c.size() : 0; }
+size_t ev_circuit_len(const ev_circuit *c) { return c ? c->c.record_count() : 0; }
 uint64_t ev_circuit_num_qubits(const ev_circuit *c) { return c ? c->c.num_qubits() : 0; }
 uint64_t ev_circuit_num_classical(const ev_circuit *c) { return c ? c->c.num_classical() : 0; }
 
@@ -145,10 +144,11 @@ int ev_circuit_export(const ev_circuit *c, ev_op *out, size_t cap, size_t *len)
     return guard([&] {
         NONNULL(c);
         NONNULL(len);
-        *len = c->c.size();
-        EV_REQUIRE(out == nullptr || cap >= c->c.size(), EV_E_ARG, "export buffer too small");
+        const size_t nrec = c->c.record_count();
+        *len = nrec;
+        EV_REQUIRE(out == nullptr || cap >= nrec, EV_E_ARG, "export buffer too small");
         if (out)
-            for (size_t i = 0; i < c->c.size(); ++i) c->c.to_ev(i, out[i]);
+            for (size_t i = 0, r = 0; i < c->c.size(); ++i) r += c->c.to_ev(i, out + r);
     });
 }
 

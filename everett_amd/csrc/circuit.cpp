@@ -2,7 +2,9 @@
 // src/algorithms/qft.rs, plus the benchmark workload generators.
 #include "circuit.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 
 namespace ev {
@@ -152,23 +154,42 @@ void Circuit::measure(uint64_t qubit, uint64_t into) { ops_.push_back(Op{OpKind:
 
 void Circuit::if_bit(uint64_t bit) { ops_.push_back(Op{OpKind::If, bit, 0, 0, 0, 0}); }
 
-void Circuit::append(const ev_op &op)
+size_t Circuit::append(const ev_op *ops, size_t count, size_t i)
 {
+    const ev_op &op = ops[i];
     const c128 *m = reinterpret_cast<const c128 *>(op.m);
     switch (op.kind) {
-    case EV_OP_APPLY1: apply1(Mat2{m[0], m[1], m[2], m[3]}, op.target); break;
+    case EV_OP_APPLY1: apply1(Mat2{m[0], m[1], m[2], m[3]}, op.target); return i + 1;
     case EV_OP_APPLY2: {
         Mat4 g;
-        for (int i = 0; i < 16; ++i) g[i] = m[i];
+        for (int k = 0; k < 16; ++k) g[k] = m[k];
         apply2(g, op.target, op.b);
-        break;
+        return i + 1;
     }
-    case EV_OP_CONTROLLED:
-        EV_REQUIRE(op.num_controls <= EV_MAX_CONTROLS, EV_E_ARG, "controlled op: num_controls exceeds EV_MAX_CONTROLS");
-        controlled(op.controls, op.num_controls, Mat2{m[0], m[1], m[2], m[3]}, op.target);
-        break;
-    case EV_OP_MEASURE: measure(op.target, op.b); break;
-    case EV_OP_IF: if_bit(op.target); break;
+    case EV_OP_CONTROLLED: {
+        // Op::Controlled { controls: Vec<QubitId>, .. } (src/op.rs:34-36) has no
+        // length limit: controls past the first EV_MAX_CONTROLS continue in the
+        // EV_OP_CONTROLS records that follow.
+        const size_t total = op.num_controls;
+        std::vector<uint64_t> cs(op.controls, op.controls + std::min<size_t>(total, EV_MAX_CONTROLS));
+        size_t j = i + 1;
+        while (cs.size() < total) {
+            EV_REQUIRE(j < count && ops[j].kind == EV_OP_CONTROLS, EV_E_ARG,
+                       "controlled op with " + std::to_string(total) + " controls: only " + std::to_string(cs.size()) +
+                           " before the next non-EV_OP_CONTROLS record");
+            const size_t k = ops[j].num_controls;
+            EV_REQUIRE(k >= 1 && k <= EV_MAX_CONTROLS && cs.size() + k <= total, EV_E_ARG,
+                       "EV_OP_CONTROLS record with a control count that does not fit its op");
+            cs.insert(cs.end(), ops[j].controls, ops[j].controls + k);
+            ++j;
+        }
+        EV_REQUIRE(total <= UINT32_MAX, EV_E_LIMIT, "more than 2^32 - 1 controls on one op");
+        controlled(cs.data(), static_cast<uint32_t>(total), Mat2{m[0], m[1], m[2], m[3]}, op.target);
+        return j;
+    }
+    case EV_OP_MEASURE: measure(op.target, op.b); return i + 1;
+    case EV_OP_IF: if_bit(op.target); return i + 1;
+    case EV_OP_CONTROLS: throw Error(EV_E_ARG, "EV_OP_CONTROLS record that does not follow a controlled op");
     default: throw Error(EV_E_ARG, "unknown op kind " + std::to_string(op.kind));
     }
 }
@@ -186,19 +207,44 @@ void Circuit::compose(const Circuit &sub)
     ctrl_pool_.insert(ctrl_pool_.end(), sub.ctrl_pool_.begin(), sub.ctrl_pool_.end());
 }
 
-void Circuit::to_ev(size_t i, ev_op &out) const
+size_t Circuit::records(size_t i) const
 {
-    std::memset(&out, 0, sizeof out);
     const Op &op = ops_[i];
-    out.kind = static_cast<uint32_t>(op.kind);
-    out.target = op.q0;
-    out.b = op.q1;
+    if (op.kind != OpKind::Controlled || op.nctrl <= EV_MAX_CONTROLS) return 1;
+    return 1 + (op.nctrl - EV_MAX_CONTROLS + EV_MAX_CONTROLS - 1) / EV_MAX_CONTROLS;
+}
+
+size_t Circuit::record_count() const
+{
+    size_t n = 0;
+    for (size_t i = 0; i < ops_.size(); ++i) n += records(i);
+    return n;
+}
+
+size_t Circuit::to_ev(size_t i, ev_op *out) const
+{
+    const size_t nrec = records(i);
+    std::memset(out, 0, nrec * sizeof(ev_op));
+    const Op &op = ops_[i];
+    out[0].kind = static_cast<uint32_t>(op.kind);
+    out[0].target = op.q0;
+    out[0].b = op.q1;
     size_t nm = op.kind == OpKind::Apply2 ? 16 : (op.kind == OpKind::Apply1 || op.kind == OpKind::Controlled ? 4 : 0);
-    std::memcpy(out.m, mat(op), nm * sizeof(c128));
+    std::memcpy(out[0].m, mat(op), nm * sizeof(c128));
     if (op.kind == OpKind::Controlled) {
-        out.num_controls = op.nctrl;
-        std::memcpy(out.controls, ctrls(op), op.nctrl * sizeof(uint64_t));
+        out[0].num_controls = op.nctrl;  // the op's total; the records carry them EV_MAX_CONTROLS at a time
+        const uint64_t *cs = ctrls(op);
+        size_t done = std::min<size_t>(op.nctrl, EV_MAX_CONTROLS);
+        std::memcpy(out[0].controls, cs, done * sizeof(uint64_t));
+        for (size_t r = 1; r < nrec; ++r) {
+            const size_t k = std::min<size_t>(op.nctrl - done, EV_MAX_CONTROLS);
+            out[r].kind = EV_OP_CONTROLS;
+            out[r].num_controls = static_cast<uint32_t>(k);
+            std::memcpy(out[r].controls, cs + done, k * sizeof(uint64_t));
+            done += k;
+        }
     }
+    return nrec;
 }
 
 static void check_qubit(uint64_t q, uint64_t nq)  // circuit.rs:288-296

@@ -74,9 +74,17 @@ class Circuit {
     void controlled(const uint64_t *controls, uint32_t nc, const Mat2 &m, uint64_t target);
     void measure(uint64_t qubit, uint64_t into);
     void if_bit(uint64_t bit);  // guards the next record
-    void append(const ev_op &op);
+    // Appends the op that starts at ops[i] (a controlled op with more than
+    // EV_MAX_CONTROLS controls continues in EV_OP_CONTROLS records) and
+    // returns the index of the record after it.
+    size_t append(const ev_op *ops, size_t count, size_t i);
     void compose(const Circuit &sub);  // circuit.rs:124-127
-    void to_ev(size_t i, ev_op &out) const;
+    // ev_op records op i takes (1, or more for a controlled op with more than
+    // EV_MAX_CONTROLS controls), and all ops together.
+    size_t records(size_t i) const;
+    size_t record_count() const;
+    // Writes op i as records(i) records from out[0]; returns that count.
+    size_t to_ev(size_t i, ev_op *out) const;
     // Register sizes set after the ops (the OpenQASM reader learns them from
     // declarations that may follow the statements).
     void set_registers(uint64_t num_qubits, uint64_t num_classical)

@@ -220,20 +220,21 @@ template <class R> __global__ void k_swap_bits(Amp<R> *psi, uint64_t len, SwapPa
 }
 
 struct CtrlArgs {
-    int npos;            // controls + target, ascending
-    int pos[EV_MAX_CONTROLS + 1];
+    uint64_t pos_mask;   // every control and the target (repeated controls are one bit)
     uint64_t ctrl_mask;  // all control bits
     uint64_t tbit;
 };
 
 // kernel.rs:332-358, enumerating only the pairs whose controls are all set:
-// insert zeros at every control and target position, then set the controls.
+// insert zeros at every control and target position (ascending, so each
+// insertion sees the bits already placed), then set the controls.  Any
+// number of controls: the positions are the set bits of a uniform mask.
 template <class R> __global__ void k_ctrl1(Amp<R> *psi, uint64_t nwork, CtrlArgs a, M2<R> g)
 {
     GRID_STRIDE(i, nwork)
     {
         uint64_t x = i;
-        for (int p = 0; p < a.npos; ++p) x = insert0(x, a.pos[p]);
+        for (uint64_t m = a.pos_mask; m; m &= m - 1) x = insert0(x, __builtin_ctzll(m));
         const uint64_t i0 = x | a.ctrl_mask, i1 = i0 | a.tbit;
         const Amp<R> a0 = psi[i0], a1 = psi[i1];
         psi[i0] = cadd(cmul(g.m[0], a0), cmul(g.m[1], a1));
@@ -509,19 +510,16 @@ template <class R> void apply2_t(void *psi, int n, int a, int b, const c128 m[16
 template <class R>
 void ctrl1_t(void *psi, int n, const uint64_t *controls, int nc, int target, const c128 m[4], hipStream_t st)
 {
+    EV_REQUIRE(n >= 1 && n <= 63 && target >= 0 && target < n, EV_E_ARG, "controlled gate: target out of range");
     CtrlArgs a{};
     a.tbit = uint64_t(1) << target;
-    std::vector<int> pos;
     for (int c = 0; c < nc; ++c) {
+        EV_REQUIRE(controls[c] < uint64_t(n) && int(controls[c]) != target, EV_E_ARG,
+                   "controlled gate: control out of range or equal to the target");
         a.ctrl_mask |= uint64_t(1) << controls[c];
-        pos.push_back(static_cast<int>(controls[c]));
     }
-    pos.push_back(target);
-    std::sort(pos.begin(), pos.end());
-    pos.erase(std::unique(pos.begin(), pos.end()), pos.end());  // repeated controls are one condition
-    a.npos = static_cast<int>(pos.size());
-    for (int i = 0; i < a.npos; ++i) a.pos[i] = pos[i];
-    const uint64_t nwork = uint64_t(1) << (n - a.npos);
+    a.pos_mask = a.ctrl_mask | a.tbit;
+    const uint64_t nwork = uint64_t(1) << (n - __builtin_popcountll(a.pos_mask));
     hipLaunchKernelGGL(k_ctrl1<R>, grid_for(nwork), dim3(kBlock), 0, st, A<R>(psi), nwork, a, to_m2<R>(m));
 }
 

@@ -66,20 +66,31 @@ const char *ev_last_error(void);
 
 /* src/op.rs:14-58.  Op::IfClassic { bit, then } is encoded as an EV_OP_IF
  * record (target = classical bit) guarding the record that follows it; nested
- * IfClassic is a run of EV_OP_IF records. */
+ * IfClassic is a run of EV_OP_IF records.
+ *
+ * Op::Controlled's `controls: Vec<QubitId>` (src/op.rs:34-36) has no length
+ * limit, and neither has this encoding: an EV_OP_CONTROLLED record's
+ * num_controls is the op's total count, its controls[] holds the first
+ * EV_MAX_CONTROLS of them, and the rest follow in EV_OP_CONTROLS records
+ * (EV_MAX_CONTROLS per record, num_controls = entries used in that record)
+ * directly after it.  A guarded controlled op is EV_OP_IF, EV_OP_CONTROLLED,
+ * then its EV_OP_CONTROLS records.  Repeated controls are allowed, as in the
+ * reference: they form one condition. */
 typedef enum {
     EV_OP_APPLY1 = 1,     /* Op::Apply1 { gate, target }               m[0..4)  */
     EV_OP_APPLY2 = 2,     /* Op::Apply2 { gate, a=target, b }          m[0..16) */
     EV_OP_CONTROLLED = 3, /* Op::Controlled { controls, gate, target } m[0..4)  */
     EV_OP_MEASURE = 4,    /* Op::Measure { qubit=target, into=b } */
-    EV_OP_IF = 5          /* Op::IfClassic { bit=target, then=<next record> } */
+    EV_OP_IF = 5,         /* Op::IfClassic { bit=target, then=<next record> } */
+    EV_OP_CONTROLS = 6    /* more controls of the EV_OP_CONTROLLED record before it */
 } ev_op_kind;
 
+/* Controls carried per record (not a limit on an op's controls). */
 #define EV_MAX_CONTROLS 16
 
 typedef struct {
     uint32_t kind;         /* ev_op_kind */
-    uint32_t num_controls; /* EV_OP_CONTROLLED: entries used in controls[] */
+    uint32_t num_controls; /* EV_OP_CONTROLLED: the op's controls in all; EV_OP_CONTROLS: entries in this record */
     uint64_t target;
     uint64_t b;
     uint64_t controls[EV_MAX_CONTROLS];
@@ -95,7 +106,7 @@ int ev_circuit_create(uint64_t num_qubits, uint64_t num_classical, ev_circuit **
 void ev_circuit_destroy(ev_circuit *c);
 int ev_circuit_append(ev_circuit *c, const ev_op *ops, size_t count);  /* circuit.rs:91-116, 200-236 */
 int ev_circuit_compose(ev_circuit *dst, const ev_circuit *src);        /* circuit.rs:124-127 */
-size_t ev_circuit_len(const ev_circuit *c);                            /* records, If prefixes included */
+size_t ev_circuit_len(const ev_circuit *c);                            /* records, If and control continuations included */
 uint64_t ev_circuit_num_qubits(const ev_circuit *c);                   /* circuit.rs:55-58 */
 uint64_t ev_circuit_num_classical(const ev_circuit *c);                /* circuit.rs:62-65 */
 int ev_circuit_export(const ev_circuit *c, ev_op *out, size_t cap, size_t *len);

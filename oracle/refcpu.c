@@ -327,9 +327,27 @@ static size_t apply_op(const backend_vt *vt, void *ctx, const ref_op *ops, size_
     case 2:
         if (execute) vt->apply_2q(ctx, op->m, op->target, op->b);
         return i + 1;
-    case 3:
-        if (execute) vt->apply_controlled(ctx, op->controls, op->num_controls, op->m, op->target);
-        return i + 1;
+    case 3: {
+        /* Op::Controlled's controls Vec (op.rs:34-36): past REF_MAX_CONTROLS
+         * they continue in kind-6 records after this one. */
+        size_t total = op->num_controls, got = total < REF_MAX_CONTROLS ? total : REF_MAX_CONTROLS, j = i + 1;
+        uint64_t *cs = (uint64_t *)malloc((total ? total : 1) * sizeof(uint64_t));
+        if (!cs) return 0;
+        memcpy(cs, op->controls, got * sizeof(uint64_t));
+        while (got < total) {
+            if (j >= nops || ops[j].kind != 6 || ops[j].num_controls < 1 || ops[j].num_controls > REF_MAX_CONTROLS ||
+                got + ops[j].num_controls > total) {
+                free(cs);
+                return 0;
+            }
+            memcpy(cs + got, ops[j].controls, ops[j].num_controls * sizeof(uint64_t));
+            got += ops[j].num_controls;
+            ++j;
+        }
+        if (execute) vt->apply_controlled(ctx, cs, total, op->m, op->target);
+        free(cs);
+        return j;
+    }
     case 4:
         if (execute) classical[op->b] = (uint8_t)vt->measure(ctx, op->target);
         return i + 1;

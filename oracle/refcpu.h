@@ -38,7 +38,9 @@ typedef struct { double re, im; } rc128;
 
 /* Same field layout as the product's ev_op so one test-side buffer feeds both.
  * kind: 1 Apply1, 2 Apply2, 3 Controlled, 4 Measure, 5 IfClassic-prefix
- * (guards the next op; src/op.rs:14-58). */
+ * (guards the next op; src/op.rs:14-58), 6 more controls of the Controlled
+ * record before it (a Controlled record's num_controls is its total; controls
+ * past the first REF_MAX_CONTROLS follow in kind-6 records). */
 #define REF_MAX_CONTROLS 16
 typedef struct {
     uint32_t kind;

@@ -112,21 +112,32 @@ def gate_set():
 
 
 def ops_array(records):
-    """Builds an ev_op-layout array from (kind, target, b, m, controls) tuples."""
+    """Builds an ev_op-layout array from (kind, target, b, m, controls) tuples.
+    A controlled op with more than 16 controls takes continuation records
+    (kind 6, 16 controls each) after its head record."""
     from oracle import OP_DTYPE
 
-    out = np.zeros(len(records), dtype=OP_DTYPE)
-    for i, (kind, target, b, m, controls) in enumerate(records):
-        out[i]["kind"] = kind
-        out[i]["target"] = target
-        out[i]["b"] = b
+    rows = []
+    for kind, target, b, m, controls in records:
+        controls = list(controls or ())
+        extra = max(0, len(controls) - 16)
+        recs = np.zeros(1 + (extra + 15) // 16, dtype=OP_DTYPE)
+        recs[0]["kind"] = kind
+        recs[0]["target"] = target
+        recs[0]["b"] = b
         if m is not None:
             mm = np.asarray(m, dtype=np.complex128).reshape(-1)
-            out[i]["m"][: mm.size] = mm
+            recs[0]["m"][: mm.size] = mm
         if controls:
-            out[i]["num_controls"] = len(controls)
-            out[i]["controls"][: len(controls)] = controls
-    return out
+            recs[0]["num_controls"] = len(controls)
+            recs[0]["controls"][: min(16, len(controls))] = controls[:16]
+            for r in range(1, len(recs)):
+                part = controls[16 * r: 16 * (r + 1)]
+                recs[r]["kind"] = 6
+                recs[r]["num_controls"] = len(part)
+                recs[r]["controls"][: len(part)] = part
+        rows.append(recs)
+    return np.concatenate(rows) if rows else np.zeros(0, dtype=OP_DTYPE)
 
 
 def adjoint_ops(ops: np.ndarray) -> np.ndarray:

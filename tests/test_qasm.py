@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 import everett_amd as ev
+from everett_amd import _ffi
 import oracle
 from oracle import qasm as oq
 
@@ -36,8 +37,8 @@ def _tuples(ops):
         if k == 2:
             return ("apply2", np.array(r["m"][:16]), int(r["target"]), int(r["b"])), i + 1
         if k == 3:
-            nc = int(r["num_controls"])
-            return ("controlled", np.array(r["m"][:4]), [int(x) for x in r["controls"][:nc]], int(r["target"])), i + 1
+            cs, j = _ffi.decode_controls(ops, i)
+            return ("controlled", np.array(r["m"][:4]), cs, int(r["target"])), j
         if k == 4:
             return ("measure", int(r["target"]), int(r["b"])), i + 1
         inner, j = one(i + 1)
