@@ -52,9 +52,11 @@ def test_samples_match_oracle_sampler_c128():
         r = oracle.Rng(seed)
         u = np.array([r.next_f64() for _ in range(512)])
         want = oracle.sample(psi, n, u)
-        # the device state differs from the oracle by <= 1e-10, so a shot can only move when u lands
-        # within that distance of a CDF boundary
-        assert np.mean(got == want) > 0.99
+        # the device state differs from the oracle by <= 1e-10, so a shot may only differ where u
+        # lands within 1e-9 of a boundary of the oracle's CDF
+        cdf = np.cumsum(np.abs(psi) ** 2)
+        for k in np.nonzero(got != want)[0]:
+            assert np.min(np.abs(cdf - u[k] * cdf[-1])) < 1e-9, (seed, k)
 
 
 def test_admission_by_memory_and_concurrent_submitters():
