@@ -214,6 +214,7 @@ PROTOTYPES = {
     "ev_sv_fidelity": (I, [P, P, C.POINTER(D)]),
     "ev_sv_bloch_vector": (I, [P, U64, P]),
     "ev_sv_sample": (I, [P, U64, U64, P]),
+    "ev_sv_sample_noalloc": (I, [P, U64, U64, P]),
     "ev_sv_run": (I, [P, U64, C.POINTER(EvRunOptions), PP, P, SZ, C.POINTER(EvError)]),
     "ev_sv_run_into": (I, [P, P, U64, C.POINTER(EvRunOptions), P, SZ, C.POINTER(EvError)]),
     "ev_sv_reset": (I, [P]),

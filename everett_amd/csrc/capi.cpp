@@ -473,6 +473,15 @@ int ev_sv_sample(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out)
     });
 }
 
+int ev_sv_sample_noalloc(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out)
+{
+    return guard([&] {
+        NONNULL(s);
+        EV_REQUIRE(shots == 0 || out != nullptr, EV_E_ARG, "out is null");
+        ev::sv_sample_noalloc(const_cast<ev_sv *>(s), shots, seed, out);
+    });
+}
+
 int ev_sv_run(const ev_circuit *c, uint64_t seed, const ev_run_options *opt, ev_sv **out_state, uint8_t *classical,
               size_t num_classical, ev_error *err)
 {

@@ -20,6 +20,7 @@ struct ev_sv {
     ev::kern::Prec prec = ev::kern::C128;
     void *psi = nullptr;            // 2^n amplitudes
     hipStream_t stream = nullptr;   // all work on this handle is ordered here
+    int device = 0;                 // the device current when the handle was made
     double *d_red = nullptr;        // reduction partials + result slots
     double *h_red = nullptr;        // pinned host copy of the result slots
     uint64_t len() const { return uint64_t(1) << n; }
@@ -56,6 +57,7 @@ void sv_read(ev_sv *s, c128 *host, uint64_t len);
 void sv_read_range(ev_sv *s, uint64_t offset, c128 *host, uint64_t len);
 void sv_write(ev_sv *s, const c128 *host, uint64_t len);
 void sv_sample(ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
+void sv_sample_noalloc(ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
 
 // Per-op device dispatch (src/backend/statevector.rs:91-107).
 void sv_apply1(ev_sv *s, int k, const c128 m[4]);

@@ -61,6 +61,12 @@ void c64_to_c128(const void *src, void *dst, uint64_t len, hipStream_t st);
 // chunks, and a per-shot inverse-CDF search.  `scan` must hold 2^n doubles
 // plus chunk totals; see sv_kernels.hip.
 size_t sample_scratch_doubles(int n);
+// Sampling through a fixed per-GPU scratch (no device allocation): the host
+// buffer holds the shots' uniforms (as doubles) on entry and their basis
+// indices on return; at most kNoAllocMaxShots shots; synchronises `st`.
+constexpr uint64_t kNoAllocMaxShots = uint64_t(1) << 16;
+void sample_noalloc(const void *psi, Prec p, int n, uint64_t shots, uint64_t *host_u_then_out, hipStream_t st,
+                    int device);
 void sample(const void *psi, Prec p, int n, const double *u, uint64_t shots, double *scratch, uint64_t *out,
             hipStream_t st);
 

@@ -4,6 +4,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,7 @@ ev_sv *sv_alloc(int n, kern::Prec p)
     s->n = n;
     s->prec = p;
     try {
+        EV_HIP(hipGetDevice(&s->device));
         EV_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         EV_HIP(hipMalloc(&s->psi, s->bytes()));
         EV_HIP(hipMalloc(&s->d_red, sizeof(double) * (kRedPartials + kRedSlots)));
@@ -209,6 +211,21 @@ void sv_sample(ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out)
         throw;
     }
     cleanup();
+}
+
+// sv_sample's result without a device allocation: the uniforms are drawn
+// into the caller's output buffer (one double per shot, in shot order), which
+// the library's fixed scratch then takes as staging in both directions.
+void sv_sample_noalloc(ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out)
+{
+    EV_REQUIRE(shots <= kern::kNoAllocMaxShots, EV_E_LIMIT,
+               "ev_sv_sample_noalloc takes at most " + std::to_string(kern::kNoAllocMaxShots) + " shots per call");
+    Rng rng = Rng::seed_from_u64(seed);
+    for (uint64_t i = 0; i < shots; ++i) {
+        const double u = rng.next_f64();
+        std::memcpy(out + i, &u, sizeof u);
+    }
+    kern::sample_noalloc(s->psi, s->prec, s->n, shots, out, s->stream, s->device);
 }
 
 void sv_apply1(ev_sv *s, int k, const c128 m[4]) { kern::apply1(s->psi, s->prec, s->n, k, m, s->stream); }

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "circuit.hpp"
@@ -679,6 +681,56 @@ void sample(const void *psi, Prec p, int n, const double *u, uint64_t shots, dou
     else
         hipLaunchKernelGGL(k_sample<float>, grid_for(shots), dim3(kBlock), 0, st, A<float>(psi), nchunks, csize, scratch, u,
                            shots, out);
+}
+
+// ------------------------------------------------ sampling without allocation
+//
+// A fixed per-GPU scratch for ev_sv_sample_noalloc: the chunk totals (at most
+// 2^16: chunk_log keeps n - chunk_log(n) <= 16), the shots' uniforms and the
+// sampled indices.  Each GPU the library runs on holds its own copy until
+// the process ends.  Host code never names the array except to look up its
+// device address (once per device, that device current); every kernel
+// argument and copy uses that address.
+namespace {
+constexpr size_t kNoAllocChunks = size_t(1) << kMaxPrefixLog;
+constexpr size_t kScratchDoubles = kNoAllocChunks + 2 * size_t(kNoAllocMaxShots);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+double *noalloc_scratch(int device)
+{
+    EV_REQUIRE(device >= 0 && device < kMaxDevices, EV_E_LIMIT, "device index beyond this build's table of 64");
+    int cur = -1;
+    EV_HIP(hipGetDevice(&cur));
+    EV_REQUIRE(cur == device, EV_E_STATE, "the state's device must be the current device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        EV_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<double *>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
+void sample_noalloc(const void *psi, Prec p, int n, uint64_t shots, uint64_t *host_u_then_out, hipStream_t st,
+                    int device)
+{
+    EV_REQUIRE(shots <= kNoAllocMaxShots, EV_E_LIMIT,
+               "ev_sv_sample_noalloc takes at most " + std::to_string(kNoAllocMaxShots) + " shots per call");
+    EV_REQUIRE(sample_scratch_doubles(n) <= kNoAllocChunks, EV_E_LIMIT, "state too large for the sampling scratch");
+    double *base = noalloc_scratch(device);
+    double *totals = base, *d_u = base + kNoAllocChunks;
+    uint64_t *d_out = reinterpret_cast<uint64_t *>(d_u + kNoAllocMaxShots);
+    // Every scratch word a call reads is written earlier in the same call
+    // (all chunk totals, then the prefix in place, the uniforms by the copy,
+    // every shot's index by k_sample), so nothing is cleared first.
+    if (shots) EV_HIP(hipMemcpyAsync(d_u, host_u_then_out, shots * sizeof(double), hipMemcpyHostToDevice, st));
+    sample(psi, p, n, d_u, shots, totals, d_out, st);
+    if (shots) EV_HIP(hipMemcpyAsync(host_u_then_out, d_out, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    EV_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace ev::kern

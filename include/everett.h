@@ -175,6 +175,18 @@ int ev_sv_bloch_vector(const ev_sv *s, uint64_t qubit, double out[3]);  /* state
 /* Born-rule sampling: `shots` basis indices, one Rng::next_f64 per shot from
  * Rng::seed_from_u64(seed), inverted through the f64 inclusive CDF of |amp|^2. */
 int ev_sv_sample(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
+/* ev_sv_sample without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same samples as
+ * ev_sv_sample.  The chunk totals, uniforms and indices live in a fixed
+ * per-GPU scratch of 1.5 MiB (held until the process ends); `out` is also
+ * the host staging buffer for the uniforms.  At most
+ * EV_SAMPLE_NOALLOC_MAX_SHOTS shots per call (EV_E_LIMIT beyond); the state's
+ * device must be current (EV_E_STATE otherwise).  Every call on one GPU
+ * shares that GPU's scratch, so calls that overlap on one GPU race; the call
+ * returns after its stream has synchronised. */
+#define EV_SAMPLE_NOALLOC_MAX_SHOTS 65536
+int ev_sv_sample_noalloc(const ev_sv *s, uint64_t shots, uint64_t seed, uint64_t *out);
 
 /* Execution options (not in the reference; defaults = zero-initialised). */
 typedef struct {
