@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "dm_ops.hpp"
@@ -255,22 +256,15 @@ class StageFuser {
     {
     }
 
-    void add(int id)
-    {
-        const GOp &op = g_[id];
-        Mat2 m2;
-        Mat4 m4;
-        int a, b;
-        if (as_single(op, &a, &m2)) return add_single(id, a, m2);
-        if (as_pair(op, &a, &b, &m4)) return add_pair(id, a, b, m4);
-        add_opaque(id);
-    }
+    void add(int id) { ids_.push_back(id); }
 
     // The fused stage as planner ops; new dense 4x4 and 2x2 coefficients go to
     // pool.  A pair block becomes one dense 4x4 when its member ops cost at
     // least as much; otherwise its members run as they are.
-    std::vector<GOp> finish(std::vector<c128> &pool) const
+    std::vector<GOp> finish(std::vector<c128> &pool)
     {
+        choose_centers();
+        for (int id : ids_) add_now(id);
         std::vector<GOp> out;
         for (const Block &bl : kron_singles()) {
             switch (bl.kind) {
@@ -322,6 +316,110 @@ class StageFuser {
     }
 
   private:
+    void add_now(int id)
+    {
+        const GOp &op = g_[id];
+        Mat2 m2;
+        Mat4 m4;
+        int a, b;
+        if (as_single(op, &a, &m2)) return add_single(id, a, m2);
+        if (as_pair(op, &a, &b, &m4)) {
+            const int oa = open_of(a);
+            const bool same_pair = oa >= 0 && oa == open_of(b) && blocks_[oa].kind == Block::Pair;
+            if (same_pair || center_.count(id)) return add_pair(id, a, b, m4);
+        }
+        add_opaque(id);
+    }
+
+    // Which two-qubit ops open a dense 4x4 block.  A block costs a 4x4 (16
+    // FMA per amplitude) whatever it absorbs, so it pays off by taking the
+    // one-qubit gates next to it: on each register qubit a run of one-qubit
+    // gates between two pair ops can join either neighbour.  In a brick-wall
+    // every run sits between two CNOTs of alternating partners; making every
+    // CNOT a block leaves each block two runs, while making every other one a
+    // block (the rest run as register swaps) gives each block four.  Greedy
+    // weighted cover: a candidate is chosen while the runs it would newly
+    // absorb cost more than the block adds over running the op bare (dense
+    // 4x4s are always blocks: they cost the same either way).
+    void choose_centers()
+    {
+        struct Edge {
+            int prev, next;
+            float w;
+            bool covered;
+        };
+        std::vector<Edge> edges;
+        std::map<int, std::vector<int>> inc;  // candidate -> incident edges
+        std::map<int, int> last;              // qubit -> last candidate on it (absent or -1: none)
+        std::map<int, int> run;               // qubit -> open run edge (absent: none)
+        auto close_run = [&](int q, int next) {
+            auto it = run.find(q);
+            if (it == run.end()) return;
+            edges[size_t(it->second)].next = next;
+            if (next >= 0) inc[next].push_back(it->second);
+            run.erase(it);
+        };
+        std::vector<int> cands;
+        for (int id : ids_) {
+            const GOp &op = g_[id];
+            Mat2 m2;
+            Mat4 m4;
+            int a, b;
+            if (as_single(op, &a, &m2)) {
+                const float w = classify1(m2.data()) == K1::Dense ? fused_cost(G_U1) : fused_cost(G_DIAG);
+                auto it = run.find(a);
+                if (it != run.end()) {
+                    edges[size_t(it->second)].w = std::max(edges[size_t(it->second)].w, w);
+                    continue;
+                }
+                auto lt = last.find(a);
+                const int prev = lt == last.end() ? -1 : lt->second;
+                run[a] = int(edges.size());
+                if (prev >= 0) inc[prev].push_back(int(edges.size()));
+                edges.push_back(Edge{prev, -1, w, false});
+                continue;
+            }
+            if (as_pair(op, &a, &b, &m4)) {
+                cands.push_back(id);
+                inc[id];
+                for (int q : {a, b}) {
+                    close_run(q, id);
+                    last[q] = id;
+                }
+                if (op.kind == G_U2) center_.insert(id);
+                continue;
+            }
+            for (uint64_t t = op.touched() & regq_; t; t &= t - 1) {  // a boundary: nothing moves across it
+                const int q = __builtin_ctzll(t);
+                close_run(q, -1);
+                last[q] = -1;
+            }
+        }
+        const float block = fused_cost(G_U2);
+        auto cover = [&](int c) {
+            for (int e : inc[c]) edges[size_t(e)].covered = true;
+        };
+        for (int c : center_) cover(c);
+        for (;;) {
+            int best = -1;
+            float best_gain = 0;
+            for (int c : cands) {
+                if (center_.count(c)) continue;
+                float w = 0;
+                for (int e : inc[c])
+                    if (!edges[size_t(e)].covered) w += edges[size_t(e)].w;
+                const float gain = w - (block - fused_cost(g_[c].kind));
+                if (gain > best_gain) {
+                    best_gain = gain;
+                    best = c;
+                }
+            }
+            if (best < 0) break;
+            center_.insert(best);
+            cover(best);
+        }
+    }
+
     bool dense_single(const Block &bl) const
     {
         return bl.kind == Block::Single && !is_identity2(bl.m2) && classify1(bl.m2.data()) == K1::Dense;
@@ -492,6 +590,8 @@ class StageFuser {
     const std::vector<GOp> &g_;
     const std::vector<c128> &pool_;
     uint64_t regq_;
+    std::vector<int> ids_;
+    std::set<int> center_;  // pair ops that open a block
     std::vector<Block> blocks_;
     std::map<int, int> open_;
 };

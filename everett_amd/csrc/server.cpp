@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -183,13 +184,26 @@ void finish(ev_server *s, const std::shared_ptr<Request> &r)
     Rng rng = Rng::seed_from_u64(r->seed);
     std::vector<double> u(r->shots);
     for (auto &x : u) x = rng.next_f64();
-    const size_t nscr = kern::sample_scratch_doubles(r->n);
+    const size_t nscr = std::max<size_t>(kern::sample_scratch_doubles(r->n), 2 * kern::kReduceBlocks + 1);
     const size_t b_scr = align256(nscr * sizeof(double)), b_u = align256(std::max<uint64_t>(1, r->shots) * 8);
     const size_t total = b_scr + 2 * b_u;
     unsigned char *scr = static_cast<unsigned char *>(s->cache.get(total));
     r->samples.assign(r->shots, 0);
     hipError_t e = hipSuccess;
-    if (r->shots) e = hipMemcpyAsync(scr + b_scr, u.data(), r->shots * 8, hipMemcpyHostToDevice, s->stream);
+    if (s->c64) {
+        // The f64 renormalisation every complex64 run ends with (the same
+        // reduction and scaling as a run on its own), so a request samples
+        // exactly as it would alone.
+        double *partials = reinterpret_cast<double *>(scr), *norm = partials + 2 * kern::kReduceBlocks;
+        double host_norm = 0.0;
+        kern::reduce_norm(r->psi, kern::C64, r->n, partials, norm, s->stream);
+        e = hipMemcpyAsync(&host_norm, norm, sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        const double nrm = std::sqrt(host_norm);
+        if (e == hipSuccess && nrm > 0.0) kern::scale(r->psi, kern::C64, r->n, 1.0 / nrm, s->stream);
+    }
+    if (r->shots && e == hipSuccess)
+        e = hipMemcpyAsync(scr + b_scr, u.data(), r->shots * 8, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) {
         kern::sample(r->psi, s->c64 ? kern::C64 : kern::C128, r->n, reinterpret_cast<double *>(scr + b_scr), r->shots,
                      reinterpret_cast<double *>(scr), reinterpret_cast<uint64_t *>(scr + b_scr + b_u), s->stream);

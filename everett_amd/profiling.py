@@ -126,6 +126,56 @@ def traffic(args) -> int:
     return 0
 
 
+SQ_COUNTERS = ("SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAIT_INST_ANY",
+               "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT")
+
+
+def sq(args) -> int:
+    """Per-dispatch SQ counters of one kernel from a `rocprofv3 --pmc SQ_...`
+    run, with per-wave figures (waves per dispatch from the kernel's grid and
+    workgroup sizes)."""
+    con = sqlite3.connect(str(args.db))
+    rows = con.execute("SELECT dispatch_id, kernel_name, counter_name, value FROM counters_collection "
+                       "ORDER BY dispatch_id").fetchall()
+    grid = {d: (gx, wx) for d, gx, wx in con.execute("SELECT dispatch_id, grid_x, workgroup_x FROM kernels")}
+    dur = {d: (e - s) * 1e-3 for d, s, e in con.execute("SELECT dispatch_id, start, end FROM kernels")}
+    con.close()
+    per: dict[int, dict[str, float]] = {}
+    for d, name, c, v in rows:
+        if args.kernel in name:
+            per.setdefault(d, {})[c] = float(v)
+    if not per:
+        raise SystemExit(f"no dispatches of a kernel matching {args.kernel!r}")
+    lines = [f"# {args.title}", "", f"command: {args.command}",
+             f"source: rocprofv3 --pmc ({Path(args.db).name}), {len(per)} dispatches of {args.kernel}", "",
+             "per dispatch (waves = grid threads / 64; SQ_WAVE_CYCLES and SQ_ACTIVE_INST_VALU count in the SQ's",
+             "own units, so their ratio is the fraction of a wave's life spent issuing VALU):", "",
+             f"{'dispatch':>8} {'us':>9} {'VALU/wave':>10} {'LDS/wave':>9} {'bankconf/wave':>14} "
+             f"{'VALU-active':>12} {'wait-any':>9}"]
+    tot = {c: 0.0 for c in SQ_COUNTERS}
+    waves_tot = 0.0
+    for d in sorted(per):
+        c = per[d]
+        gx, _ = grid.get(d, (0, 64))
+        waves = gx / 64.0
+        waves_tot += waves
+        for k in SQ_COUNTERS:
+            tot[k] += c.get(k, 0.0)
+        lines.append(f"{d:>8} {dur.get(d, 0.0):>9.1f} {c['SQ_INSTS_VALU'] / waves:>10.0f} "
+                     f"{c['SQ_INSTS_LDS'] / waves:>9.0f} {c['SQ_LDS_BANK_CONFLICT'] / waves:>14.0f} "
+                     f"{c['SQ_ACTIVE_INST_VALU'] / c['SQ_WAVE_CYCLES']:>12.3f} "
+                     f"{c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:>9.3f}")
+    lines += ["", "all dispatches: " + ", ".join(f"{k} {tot[k]:.4g}" for k in SQ_COUNTERS),
+              f"VALU instructions per wave {tot['SQ_INSTS_VALU'] / waves_tot:.0f}, "
+              f"LDS instructions per wave {tot['SQ_INSTS_LDS'] / waves_tot:.0f}, "
+              f"VALU-active fraction of wave cycles {tot['SQ_ACTIVE_INST_VALU'] / tot['SQ_WAVE_CYCLES']:.3f}, "
+              f"waiting on any instruction {tot['SQ_WAIT_INST_ANY'] / tot['SQ_WAVE_CYCLES']:.3f}"]
+    text = "\n".join(lines) + "\n"
+    Path(args.out).write_text(text)
+    print(text, end="")
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -147,8 +197,14 @@ def main(argv=None) -> int:
     w.add_argument("--json", required=True)
     w.add_argument("--title", default="HBM traffic per launch")
     w.add_argument("--out", required=True)
+    q = sub.add_parser("sq")
+    q.add_argument("--db", required=True)
+    q.add_argument("--command", required=True)
+    q.add_argument("--kernel", required=True, help="substring of the kernel name")
+    q.add_argument("--title", default="SQ counters per dispatch")
+    q.add_argument("--out", required=True)
     args = ap.parse_args(argv)
-    return trace(args) if args.cmd == "trace" else traffic(args)
+    return {"trace": trace, "traffic": traffic, "sq": sq}[args.cmd](args)
 
 
 if __name__ == "__main__":
