@@ -34,7 +34,11 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "gates/sec + HBM GB/s at 30q f64; served circuits/sec with p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
-VALU_PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X published FP64 / FP32 vector peaks
+# MI355X vector peaks: FP64 78.6 TFLOP/s; FP32 157.3 TFLOP/s only with packed FP32 math, which this
+# build compiles without, so its FP32 vector rate is the unpacked one, equal to FP64's
+VALU_PEAK_TFLOPS = {"c128": 78.6, "c64": 78.6}
+VALU_BOUND = {"c128": "valu-fp64", "c64": "valu-fp32 (unpacked)"}
+HOST_THREADS = min(16, os.cpu_count() or 1)  # the host share of one GPU on the GPU box
 
 
 def log(msg: str) -> None:
@@ -70,6 +74,59 @@ def cpu_baseline(circuit, n: int, seconds: float) -> dict:
         "sample": f"gates 2..{count + 1} of the same {n}-qubit circuit on a host complex128 state "
                   f"(gate 1 untimed, pages the state in), {dt:.1f} s, oracle/refcpu.c single thread",
     }
+
+
+def cpu_baseline_all_cores(circuit, n: int, seconds: float) -> dict:
+    """The same oracle loops on all host cores (OpenMP over the reduction-free gate loops, so bit
+    for bit the 1-thread result): a labelled restatement timing, not the reference's."""
+    import numpy as np
+
+    import oracle
+
+    oracle.set_threads(HOST_THREADS)
+    ops = circuit.ops()
+    amps = np.zeros(1 << n, dtype=np.complex128)
+    amps[0] = 1.0
+    oracle.sv_apply_ops(ops[0:1], amps)
+    count, i = 0, 1
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and i < len(ops):
+        oracle.sv_apply_ops(ops[i:i + 1], amps)
+        i += 1
+        count += 1
+    dt = time.perf_counter() - t0
+    oracle.set_threads(1)
+    del amps
+    return {"value": count / dt, "unit": "gates/s", "cores": HOST_THREADS, "kind": "port",
+            "label": f"restatement, {HOST_THREADS} cores -- not the reference",
+            "sample": f"gates 2..{count + 1} of the same {n}-qubit circuit, {dt:.1f} s, oracle/refcpu.c with "
+                      f"OpenMP on {HOST_THREADS} threads"}
+
+
+def oracle_fidelity(circuits, idxs, state_of) -> dict:
+    """Per-circuit fidelity and max-abs of complex64 states against the FP64 oracle (all host cores)."""
+    import numpy as np
+
+    import oracle
+
+    oracle.set_threads(HOST_THREADS)
+    infid, mabs, sizes = [], [], []
+    t0 = time.perf_counter()
+    for i in idxs:
+        c = circuits[i]
+        n = c.num_qubits()
+        want, _ = oracle.sv_run(c.ops(), n)
+        got = state_of(i)
+        f = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+        infid.append(1.0 - f)
+        mabs.append(float(np.max(np.abs(got - want))))
+        sizes.append(n)
+    oracle.set_threads(1)
+    return {"reference": f"FP64 oracle (oracle/refcpu.c, OpenMP on {HOST_THREADS} host threads)",
+            "circuits": len(idxs), "qubits": [min(sizes), max(sizes)] if sizes else None,
+            "max_infidelity": float(max(infid)) if infid else None,
+            "median_infidelity": float(np.median(infid)) if infid else None,
+            "max_abs": float(max(mabs)) if mabs else None, "oracle_s": time.perf_counter() - t0}
 
 
 def run_sv_workload(circuit, n, precision, steps, warmup, plain):
@@ -142,9 +199,10 @@ def compute_roofline(agg, pass_flops, precision):
         return None
     per_launch = pass_flops / fp["launches"]
     avg_ms = fp["total_ms"] / fp["launches"]
-    peak = VALU_PEAK_TFLOPS["c64" if precision == "c64" else "c128"]
+    key = "c64" if precision == "c64" else "c128"
+    peak = VALU_PEAK_TFLOPS[key]
     achieved = per_launch / (avg_ms * 1e-3) / 1e12
-    return {"bound": "valu-" + ("fp32" if precision == "c64" else "fp64"), "kernel": "fused_pass",
+    return {"bound": VALU_BOUND[key], "kernel": "fused_pass",
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "flops_per_launch": per_launch}
 
@@ -179,10 +237,12 @@ def bench_c2(args) -> dict:
     ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 13)  # warmup
     t0 = time.perf_counter()
     ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 13)
+    rs = ev.last_run_stats()
     rho = ex.density_matrix()
     tr = rho.trace()
     wall = time.perf_counter() - t0
-    rs = ev.last_run_stats()
+    ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, 13, profile=True)  # per-launch times for the roofline
+    agg = {ks["name"]: ks for ks in ev.last_kernel_stats()}
     gates = rs.logical_gates
     pass_bytes = 2 * (4 ** 13) * 16
     return {
@@ -196,6 +256,7 @@ def bench_c2(args) -> dict:
         "gbps_per_pass": (pass_bytes * rs.passes) / (rs.device_ms * 1e-3) / 1e9 if rs.device_ms else None,
         "trace": tr,
         "classical": "".join("1" if b else "0" for b in ex.classical()),
+        "roofline": roofline(agg, 1, "c2", 26, "c128") if agg else None,
     }
 
 
@@ -220,6 +281,19 @@ def bench_c3(args) -> dict:
     x = ev.Batch.run(cs, precision="c64", use_graph=True, renormalize=True)
     wall = time.perf_counter() - t0
     st = x.stats()
+    rounds_ms = st["device_ms"]
+    achieved = st["pass_bytes"] / (rounds_ms * 1e-3) / 1e9
+    # FP64 oracle on a stated subset: per qubit count 10..22 the first circuits (in index order)
+    # with that count, 6 each up to 17 qubits, 3 at 18-19 and 2 at 20-22
+    want = {n: (6 if n <= 17 else 3 if n <= 19 else 2) for n in range(10, 23)}
+    sub = []
+    for i, c in enumerate(cs):
+        n = c.num_qubits()
+        if want.get(n, 0) > 0:
+            want[n] -= 1
+            sub.append(i)
+    fid_oracle = oracle_fidelity(cs, sub, lambda i: x.amplitudes(i))
+    fid_oracle["subset"] = "per qubit count 10..22 the first circuits in index order: 6 each up to 17 qubits, 3 at 18-19, 2 at 20-22"
     ref = ev.Batch.run(cs, precision="c128", use_graph=True)
     f, m = x.compare(ref)
     infid = 1.0 - f
@@ -241,8 +315,15 @@ def bench_c3(args) -> dict:
         "passes": st["passes"],
         "graph": bool(st["graph"]),
         "autotune_s": tune_s,
-        "fidelity_vs_f64": {"max_infidelity": float(infid.max()), "median_infidelity": float(np.median(infid)),
-                            "max_abs": float(m.max()), "reference": "the same circuits in complex128 on the GPU"},
+        "fidelity_vs_f64": fid_oracle,
+        "fidelity_vs_gpu_c128": {"circuits": count, "max_infidelity": float(infid.max()),
+                                 "median_infidelity": float(np.median(infid)), "max_abs": float(m.max()),
+                                 "reference": "the same circuits in complex128 on the GPU (all of them)"},
+        "roofline": {"bound": "hbm", "kernel": "fused_batch", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "bytes": st["pass_bytes"],
+                     "avg_launch_ms": rounds_ms / max(1, st["rounds"]), "launches": st["rounds"],
+                     "note": "algorithmic bytes of all rounds (every pass reads and writes its state once) over the "
+                             "HIP-event time of the rounds"},
     }
 
 
@@ -294,6 +375,13 @@ def bench_c4(args) -> dict:
     span_s = (max(i["done_ms"] for i in infos) - min(i["submit_ms"] for i in infos)) * 1e-3
     served = R / span_s
 
+    # the largest request on an idle server: its service time bounds the tail of the stream from below
+    big = max(range(R), key=lambda k: (reqs[k][0].num_qubits(), len(reqs[k][0])))
+    with ev.Server(precision="c64") as srv:
+        t2 = time.perf_counter()
+        srv.wait(srv.submit(reqs[big][0], shots=1024, seed=reqs[big][1]))
+        big_alone_ms = (time.perf_counter() - t2) * 1e3
+
     # single-request plain-GPU baseline: the same requests one at a time, one launch per gate
     t1 = time.perf_counter()
     done = 0
@@ -305,9 +393,19 @@ def bench_c4(args) -> dict:
             break
     plain_rate = done / (time.perf_counter() - t1)
 
-    # c64 fidelity against complex128 for a sample of the served circuits
-    sample = [c for c, _ in reqs if c.num_qubits() <= 22][:64]
-    f, m = ev.Batch.run(sample, precision="c64").compare(ev.Batch.run(sample, precision="c128"))
+    # c64 fidelity against the FP64 oracle: the first 16 requests of at most 20 qubits, run on their own
+    # (a served request samples exactly as it does on its own)
+    circs = [c for c, _ in reqs]
+    sub = [i for i, c in enumerate(circs) if c.num_qubits() <= 20][:16]
+    fid = oracle_fidelity(circs, sub, lambda i: ev.StateVectorBackend.run(circs[i], precision="c64").state().amplitudes())
+    fid["subset"] = "the first 16 requests (arrival order) with at most 20 qubits, run alone in complex64"
+    by_n = {}
+    ns = np.array([c.num_qubits() for c in circs])
+    for lo, hi in ((12, 15), (16, 19), (20, 23), (24, 28)):
+        sel = (ns >= lo) & (ns <= hi)
+        if sel.any():
+            by_n[f"{lo}-{hi}"] = {"requests": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                                  "p99_ms": float(np.percentile(lat[sel], 99))}
     return {
         "workload": f"{R} requests, n = round(exp(U[ln 12, ln 28])), depth 20..200 brick-wall, 1024 shots each, "
                     f"Poisson arrivals at {rate:.2f}/s ({args.serve_load:.0%} of the burst capacity), complex64",
@@ -323,8 +421,10 @@ def bench_c4(args) -> dict:
         "busy_ms": st["busy_ms"],
         "bytes_peak": st["bytes_peak"],
         "wall_s": wall,
-        "c64_vs_c128_sample": {"circuits": len(sample), "max_infidelity": float((1 - f).max()),
-                               "max_abs": float(m.max())},
+        "latency_by_qubits": by_n,
+        "largest_request_alone_ms": big_alone_ms,
+        "largest_request": f"request {big}: {reqs[big][0].num_qubits()} qubits, {len(reqs[big][0])} gates, on an idle server",
+        "fidelity_vs_f64": fid,
     }
 
 
@@ -398,7 +498,16 @@ def main() -> int:
         if not args.no_cpu:
             log(f"c1 cpu baseline ({args.cpu_seconds:.0f} s sample)")
             out["cpu_baseline"] = cpu_baseline(c1, n1, args.cpu_seconds)
+            log(f"c1 cpu restatement on {HOST_THREADS} cores ({args.cpu_seconds / 2:.0f} s sample)")
+            out["cpu_restatement_all_cores"] = cpu_baseline_all_cores(c1, n1, args.cpu_seconds / 2)
     extra = {}
+    if wl == "all" and args.precision == "c128" and not args.plain:
+        c1 = ev.workloads.config1(n1, 40)
+        r = bench_sv("c1_c64", c1, n1, f"{n1}-qubit brick-wall depth 40 (seed {n1}) + QFT({n1}), complex64",
+                     argparse.Namespace(**dict(vars(args), precision="c64")), steps, warmup)
+        extra["c1_c64"] = {k: r[k] for k in ("workload", "gates", "gates_per_s", "ms_per_step", "device_ms_per_step",
+                                             "passes_per_step", "roofline", "compute_roofline", "kernels")}
+        log(f"c1 c64: {r['gates_per_s']:.0f} gates/s")
     if wl in ("all", "c0"):
         c0 = ev.workloads.config0(20, 20)
         r = bench_sv("c0", c0, 20, "20-qubit brick-wall depth 20 (seed 20)", args, 10, 2)

@@ -209,8 +209,8 @@ class DensityMatrixBackend:
     @staticmethod
     def run_with_noise_seeded(circuit: Circuit, noise: NoiseModel, seed: int, plain: bool = False,
                               tile_qubits: int = 0, max_gates_per_pass: int = 0,
-                              low_qubits: int = 0) -> DensityMatrixExecution:
-        opt = run_options("c128", plain, tile_qubits, max_gates_per_pass, low_qubits=low_qubits)
+                              low_qubits: int = 0, profile: bool = False) -> DensityMatrixExecution:
+        opt = run_options("c128", plain, tile_qubits, max_gates_per_pass, low_qubits=low_qubits, profile=profile)
         keep: list = []
         nm = noise._c(keep)
         ncl = circuit.num_classical()

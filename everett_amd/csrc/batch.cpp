@@ -4,8 +4,8 @@
 // pipelined groups: each group is planned into fused passes (in parallel on
 // host threads) while the device runs the previous group; round r of a
 // group is ONE launch over every tile of the r-th pass of every circuit of
-// the group that has one, so small and large states share the GPU.  A
-// one-group batch can run its rounds as one captured HIP graph.
+// the group that has one, so small and large states share the GPU.  Each
+// group's rounds can run as one captured HIP graph.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -95,18 +95,21 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
     b->prec = c64 ? kern::C64 : kern::C128;
     std::vector<void *> pinned;
     std::vector<hipEvent_t> events;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> gexecs;
+    hipStream_t cap = nullptr;  // an idle stream the groups' rounds are captured on
     auto cleanup = [&] {
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         for (void *h : pinned) (void)hipHostFree(h);
         pinned.clear();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
+        for (hipGraphExec_t x : gexecs) (void)hipGraphExecDestroy(x);
+        for (hipGraph_t x : graphs) (void)hipGraphDestroy(x);
+        gexecs.clear();
+        graphs.clear();
+        if (cap) (void)hipStreamDestroy(cap);
+        cap = nullptr;
     };
     try {
         // ---- validate (on host threads; the first failing circuit in index
@@ -183,7 +186,12 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
         batchk::init(b->arena, c64, amps, b->d_meta, int(count), b->stream);
 
         char *abase = static_cast<char *>(b->arena);
-        const bool graph_ok = opt.use_graph && ngroups == 1;
+        // Each group's rounds can run as one captured HIP graph.  They are
+        // captured on an idle side stream (capture records the launches
+        // without running them) and the graph is launched on the batch's
+        // stream, so capturing group g+1 does not wait for group g to finish.
+        const bool graph_ok = opt.use_graph != 0;
+        if (graph_ok) EV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         for (size_t g = 0; g < ngroups; ++g) {
             const size_t i0 = gbeg[g], i1 = gbeg[g + 1], gc = i1 - i0;
             if (gc == 0) continue;
@@ -215,6 +223,7 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
                 loc[k].npass = uint32_t(P.passes.size());
                 max_passes = std::max(max_passes, loc[k].npass);
                 b->stats.passes += P.passes.size();
+                b->stats.pass_bytes += double(P.passes.size()) * 2.0 * double(uint64_t(1) << b->n[i0 + k]) * double(ab);
             }
             const size_t entries_off = off;
             std::vector<size_t> round_off(max_passes), round_n(max_passes);
@@ -272,6 +281,7 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
                     en.ops = reinterpret_cast<const fused::DevOp *>(dbase + loc[k].op_off);
                     en.coef = dbase + loc[k].coef_off;
                     en.tile_begin = tiles;
+                    en.tile_off = 0;
                     tiles += uint64_t(1) << (b->n[i] - ps.t);
                     round_t[r] = std::max(round_t[r], int(ps.t));
                 }
@@ -282,28 +292,31 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             EV_HIP(hipMemcpyAsync(dev, host, desc_bytes, hipMemcpyHostToDevice, b->stream));
             b->stats.upload_ms += now_ms() - t_up;
 
-            // ---- the group's rounds (one captured graph when the batch is one group)
+            // ---- the group's rounds (one captured graph per group when asked)
             const fused::BatchEntry *dent = reinterpret_cast<const fused::BatchEntry *>(dbase + entries_off);
-            auto enqueue = [&] {
+            auto enqueue = [&](hipStream_t st) {
                 for (uint32_t r = 0; r < max_passes; ++r)
-                    fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], round_t[r], b->stream);
+                    fused::launch_batch(c64, dent + round_off[r], int(round_n[r]), round_tiles[r], round_t[r], st);
             };
             hipEvent_t e0, e1;
             EV_HIP(hipEventCreate(&e0));
             events.push_back(e0);
             EV_HIP(hipEventCreate(&e1));
             events.push_back(e1);
-            if (graph_ok) {
-                EV_HIP(hipStreamSynchronize(b->stream));  // capture starts on an idle stream
-                EV_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                enqueue();
-                EV_HIP(hipStreamEndCapture(b->stream, &graph));
+            hipGraphExec_t gexec = nullptr;
+            if (graph_ok && max_passes > 0) {
+                hipGraph_t graph = nullptr;
+                EV_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                enqueue(cap);
+                EV_HIP(hipStreamEndCapture(cap, &graph));
+                graphs.push_back(graph);
                 EV_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                gexecs.push_back(gexec);
                 b->stats.graph = 1;
             }
             EV_HIP(hipEventRecord(e0, b->stream));
             if (gexec) EV_HIP(hipGraphLaunch(gexec, b->stream));
-            else enqueue();
+            else enqueue(b->stream);
             EV_HIP(hipEventRecord(e1, b->stream));
             b->stats.rounds += max_passes;
             ++b->stats.groups;

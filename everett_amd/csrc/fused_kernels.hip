@@ -590,7 +590,8 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
 }
 
 // Batch kernel: workgroup b runs linear tile b of the round, whose entry
-// (entries sorted by tile_begin) is found by binary search.
+// (entries sorted by tile_begin) is found by binary search; an entry covers
+// the tiles [tile_off, tile_off + its count) of its pass.
 template <class R>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_batch(
     const BatchEntry *__restrict__ entries, int nentries)
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
     }
     const cptr<BatchEntry> e = ent + lo;
     one_tile<R>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
-                            as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin),
+                            as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin + e->tile_off),
                 lds_raw);
 }
 

@@ -14,8 +14,10 @@ namespace ev::fused {
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
                  const void *coef, hipStream_t st);
 
-// One entry of a batched round: the next pass of one state.  tile_begin is the
-// prefix sum of the tile counts of the entries before it.
+// One entry of a batched round: a run of consecutive tiles of one state's next
+// pass, from tile tile_off on.  tile_begin is the prefix sum of the tile
+// counts of the entries before it, so linear tile b of the launch is tile
+// b - tile_begin + tile_off of the entry with the last tile_begin <= b.
 struct BatchEntry {
     void *psi;
     const Pass *pass;
@@ -23,6 +25,7 @@ struct BatchEntry {
     const DevOp *ops;
     const void *coef;
     uint64_t tile_begin;
+    uint64_t tile_off;
 };
 
 // One launch over every entry of a round: total_tiles workgroups of

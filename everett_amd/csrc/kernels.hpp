@@ -43,6 +43,9 @@ void reduce_bloch(const void *psi, Prec p, int n, int k, double *partials, doubl
 void collapse(void *psi, Prec p, int n, int k, int outcome, double scale, hipStream_t st);
 // src/state.rs:103-111: multiply every amplitude by s.
 void scale(void *psi, Prec p, int n, double s, hipStream_t st);
+// src/state.rs:103-111 on the device: <psi|psi> into *norm_slot (partials as
+// for reduce_norm), then psi /= sqrt of it when positive; no host round trip.
+void normalize(void *psi, Prec p, int n, double *partials, double *norm_slot, hipStream_t st);
 
 // Streaming copy of `bytes` (a multiple of 32), the HBM bandwidth reference.
 void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);

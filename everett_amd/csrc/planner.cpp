@@ -1423,7 +1423,7 @@ void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng,
 {
     ev_sv *v = d->vec;
     const int n = d->n;
-    Profiler prof(v->stream, false);
+    Profiler prof(v->stream, opt.profile != 0);
     if (2 * n < fused::kMinTile) {
         size_t i = 0;
         while (i < c.size()) i = dm_plain_op(d, c, i, nz, rng, classical, true, st);
@@ -1497,6 +1497,7 @@ void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng,
         }
     }
     flush_segment(v, low, cfg, st, prof);
+    st.kernels = prof.collect();
 }
 
 }  // namespace ev

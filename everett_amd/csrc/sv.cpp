@@ -120,11 +120,7 @@ bool sv_measure(ev_sv *s, int q, double u)
 }
 
 // src/state.rs:103-111
-void sv_normalize(ev_sv *s)
-{
-    const double norm = std::sqrt(sv_norm_sqr(s));
-    if (norm > 0.0) kern::scale(s->psi, s->prec, s->n, 1.0 / norm, s->stream);
-}
+void sv_normalize(ev_sv *s) { kern::normalize(s->psi, s->prec, s->n, s->d_red, slots(s), s->stream); }
 
 void sv_read(ev_sv *s, c128 *host, uint64_t len)
 {

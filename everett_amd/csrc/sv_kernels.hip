@@ -350,6 +350,21 @@ template <class R> __global__ void k_scale(Amp<R> *psi, uint64_t len, double s)
     }
 }
 
+// src/state.rs:103-111 with the norm read from device memory: scale by
+// 1 / sqrt(<psi|psi>) when it is positive (IEEE sqrt and division, the
+// values the host formula gives).
+template <class R> __global__ void k_scale_by_norm(Amp<R> *psi, uint64_t len, const double *norm_sqr)
+{
+    const double x = *norm_sqr;
+    if (!(x > 0.0)) return;
+    const double s = 1.0 / __builtin_sqrt(x);
+    GRID_STRIDE(j, len)
+    {
+        const Amp<R> a = psi[j];
+        psi[j] = mk<Amp<R>>(R(a.x * s), R(a.y * s));
+    }
+}
+
 __global__ void k_c128_to_c64(const double2 *src, float2 *dst, uint64_t len)
 {
     GRID_STRIDE(j, len) dst[j] = make_float2(float(src[j].x), float(src[j].y));
@@ -635,6 +650,16 @@ void scale(void *psi, Prec p, int n, double s, hipStream_t st)
     const uint64_t len = uint64_t(1) << n;
     if (p == C128) hipLaunchKernelGGL(k_scale<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, s);
     else hipLaunchKernelGGL(k_scale<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, s);
+}
+
+void normalize(void *psi, Prec p, int n, double *partials, double *norm_slot, hipStream_t st)
+{
+    reduce_norm(psi, p, n, partials, norm_slot, st);
+    const uint64_t len = uint64_t(1) << n;
+    if (p == C128)
+        hipLaunchKernelGGL(k_scale_by_norm<double>, grid_for(len), dim3(kBlock), 0, st, A<double>(psi), len, norm_slot);
+    else
+        hipLaunchKernelGGL(k_scale_by_norm<float>, grid_for(len), dim3(kBlock), 0, st, A<float>(psi), len, norm_slot);
 }
 
 void c128_to_c64(const void *src, void *dst, uint64_t len, hipStream_t st)

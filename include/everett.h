@@ -195,8 +195,8 @@ typedef struct {
     uint32_t tile_qubits;   /* fused pass tile width, 8..13 (13 = 512-thread workgroups); 0 = default */
     uint32_t max_gates_per_pass; /* 0 = default */
     uint32_t profile;       /* 1: time every kernel launch with HIP events (ev_last_run_kernel_stats) */
-    uint32_t use_graph;     /* batch: 1 = capture the rounds in a HIP graph and launch it once
-                               (batches small enough to run as one pipelined group) */
+    uint32_t use_graph;     /* batch: 1 = capture each pipelined group's rounds in a HIP graph and
+                               launch it once */
     uint32_t renormalize;   /* batch: rescale every state to unit f64 norm at the end (c64 drift) */
     uint32_t low_qubits;    /* lowest qubits always in a tile (contiguous rows); 0 = tuned or default */
 } ev_run_options;
@@ -284,8 +284,10 @@ typedef struct ev_batch ev_batch;
 typedef struct {
     uint64_t circuits, logical_gates, rounds, passes, tiles;
     double plan_ms, upload_ms, device_ms, renorm_ms;
-    uint32_t graph;  /* 1 if the rounds ran as one captured HIP graph (one-group batches) */
+    uint32_t graph;  /* 1 if the groups' rounds ran as captured HIP graphs (one per group) */
     uint32_t groups; /* pipelined groups: the host plans group g+1 while the device runs group g */
+    double pass_bytes; /* algorithmic HBM bytes of all fused passes: per pass, every amplitude of its
+                          state read and written once (2 * 2^n * amplitude size) */
 } ev_batch_stats;
 int ev_batch_run(const ev_circuit *const *circuits, size_t count, const ev_run_options *opt, ev_batch **out,
                  ev_error *err);
