@@ -10,7 +10,11 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libeverett.so"
+import os
+
+# EVERETT_AMD_LIB selects another build of the same library (the host-sanitized
+# one, everett_amd/build.py --sanitize); the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("EVERETT_AMD_LIB") or Path(__file__).resolve().parent / "_lib" / "libeverett.so")
 
 MAX_CONTROLS = 16  # controls carried per ev_op record (an op's controls continue in OP_CONTROLS records)
 

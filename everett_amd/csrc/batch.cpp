@@ -250,14 +250,17 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             unsigned char *dbase = static_cast<unsigned char *>(dev);
             parallel_for(0, gc, [&](size_t k) {
                 const Plan &P = plans[k];
-                std::memcpy(host + loc[k].pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
-                std::memcpy(host + loc[k].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
+                if (!P.passes.empty())
+                    std::memcpy(host + loc[k].pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
+                if (!P.stages.empty())
+                    std::memcpy(host + loc[k].stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
                 if (!P.devops.empty())
                     std::memcpy(host + loc[k].op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
                 if (c64)
                     fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host + loc[k].op_off), P.devops.size());
                 if (!c64) {
-                    std::memcpy(host + loc[k].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
+                    if (!P.coef.empty())
+                        std::memcpy(host + loc[k].coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
                 } else {
                     float *f = reinterpret_cast<float *>(host + loc[k].coef_off);
                     for (size_t j = 0; j < P.coef.size(); ++j) {

@@ -963,10 +963,10 @@ int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, 
         info->stage_bytes = sizeof(ev::fused::Stage);
         info->op_bytes = sizeof(ev::fused::DevOp);
         info->tile_qubits = b.t;
-        if (passes) std::memcpy(passes, b.passes.data(), b.passes.size());
-        if (stages) std::memcpy(stages, b.stages.data(), b.stages.size());
-        if (ops) std::memcpy(ops, b.ops.data(), b.ops.size());
-        if (coef) std::memcpy(coef, b.coef.data(), b.coef.size() * sizeof(ev_c128));
+        if (passes && !b.passes.empty()) std::memcpy(passes, b.passes.data(), b.passes.size());
+        if (stages && !b.stages.empty()) std::memcpy(stages, b.stages.data(), b.stages.size());
+        if (ops && !b.ops.empty()) std::memcpy(ops, b.ops.data(), b.ops.size());
+        if (coef && !b.coef.empty()) std::memcpy(coef, b.coef.data(), b.coef.size() * sizeof(ev_c128));
     });
 }
 

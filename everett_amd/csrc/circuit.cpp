@@ -230,8 +230,8 @@ size_t Circuit::to_ev(size_t i, ev_op *out) const
     out[0].target = op.q0;
     out[0].b = op.q1;
     size_t nm = op.kind == OpKind::Apply2 ? 16 : (op.kind == OpKind::Apply1 || op.kind == OpKind::Controlled ? 4 : 0);
-    std::memcpy(out[0].m, mat(op), nm * sizeof(c128));
-    if (op.kind == OpKind::Controlled) {
+    if (nm) std::memcpy(out[0].m, mat(op), nm * sizeof(c128));
+    if (op.kind == OpKind::Controlled && op.nctrl) {
         out[0].num_controls = op.nctrl;  // the op's total; the records carry them EV_MAX_CONTROLS at a time
         const uint64_t *cs = ctrls(op);
         size_t done = std::min<size_t>(op.nctrl, EV_MAX_CONTROLS);

@@ -1228,12 +1228,12 @@ void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
     const size_t o_coef = o_ops + align256(std::max<size_t>(1, P.devops.size()) * sizeof(DevOp));
     const size_t total = o_coef + align256(std::max<size_t>(1, P.coef.size()) * ab);
     std::vector<unsigned char> host(total, 0);
-    std::memcpy(host.data() + o_pass, P.passes.data(), P.passes.size() * sizeof(Pass));
-    std::memcpy(host.data() + o_stage, P.stages.data(), P.stages.size() * sizeof(Stage));
+    std::memcpy(host.data() + o_pass, P.passes.data(), P.passes.size() * sizeof(Pass));  // P.passes is not empty
+    if (!P.stages.empty()) std::memcpy(host.data() + o_stage, P.stages.data(), P.stages.size() * sizeof(Stage));
     if (!P.devops.empty()) std::memcpy(host.data() + o_ops, P.devops.data(), P.devops.size() * sizeof(DevOp));
     if (c64) fused::inline_coef_to_c64(reinterpret_cast<DevOp *>(host.data() + o_ops), P.devops.size());
     if (!c64) {
-        std::memcpy(host.data() + o_coef, P.coef.data(), P.coef.size() * sizeof(c128));
+        if (!P.coef.empty()) std::memcpy(host.data() + o_coef, P.coef.data(), P.coef.size() * sizeof(c128));
     } else {
         float *f = reinterpret_cast<float *>(host.data() + o_coef);
         for (size_t i = 0; i < P.coef.size(); ++i) {

@@ -229,12 +229,13 @@ void admit(ev_server *s, Request &r)
     r.desc_bytes = off;
     std::vector<unsigned char> &host = r.desc_host;
     host.assign(off, 0);
-    std::memcpy(host.data() + r.pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
-    std::memcpy(host.data() + r.stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
+    if (!P.passes.empty()) std::memcpy(host.data() + r.pass_off, P.passes.data(), P.passes.size() * sizeof(fused::Pass));
+    if (!P.stages.empty())
+        std::memcpy(host.data() + r.stage_off, P.stages.data(), P.stages.size() * sizeof(fused::Stage));
     if (!P.devops.empty()) std::memcpy(host.data() + r.op_off, P.devops.data(), P.devops.size() * sizeof(fused::DevOp));
     if (s->c64) fused::inline_coef_to_c64(reinterpret_cast<fused::DevOp *>(host.data() + r.op_off), P.devops.size());
     if (!s->c64) {
-        std::memcpy(host.data() + r.coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
+        if (!P.coef.empty()) std::memcpy(host.data() + r.coef_off, P.coef.data(), P.coef.size() * sizeof(c128));
     } else {
         float *f = reinterpret_cast<float *>(host.data() + r.coef_off);
         for (size_t k = 0; k < P.coef.size(); ++k) {

@@ -17,8 +17,11 @@ from pathlib import Path
 
 import numpy as np
 
+import os
+
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "build" / "liboracle.so"
+# ORACLE_LIB selects the sanitized build (make -C oracle sanitize) for host checks
+LIB_PATH = Path(os.environ.get("ORACLE_LIB") or HERE / "build" / "liboracle.so")
 
 OP_DTYPE = np.dtype(
     [
