@@ -166,7 +166,8 @@ class EvPlanInfo(C.Structure):
     _fields_ = [
         ("num_passes", C.c_uint32), ("num_stages", C.c_uint32), ("num_ops", C.c_uint32), ("num_coef", C.c_uint32),
         ("pass_bytes", C.c_uint32), ("stage_bytes", C.c_uint32), ("op_bytes", C.c_uint32),
-        ("tile_qubits", C.c_uint32),
+        ("tile_qubits", C.c_uint32), ("num_swap_pairs", C.c_uint32),
+        ("swap_a", C.c_uint32 * 32), ("swap_b", C.c_uint32 * 32),
     ]
 
 

@@ -963,6 +963,12 @@ int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, 
         info->stage_bytes = sizeof(ev::fused::Stage);
         info->op_bytes = sizeof(ev::fused::DevOp);
         info->tile_qubits = b.t;
+        EV_REQUIRE(b.swap_tail.size() <= 32, EV_E_LIMIT, "swap tail longer than ev_plan_info holds");
+        info->num_swap_pairs = uint32_t(b.swap_tail.size());
+        for (size_t k = 0; k < b.swap_tail.size(); ++k) {
+            info->swap_a[k] = uint32_t(b.swap_tail[k].first);
+            info->swap_b[k] = uint32_t(b.swap_tail[k].second);
+        }
         if (passes && !b.passes.empty()) std::memcpy(passes, b.passes.data(), b.passes.size());
         if (stages && !b.stages.empty()) std::memcpy(stages, b.stages.data(), b.stages.size());
         if (ops && !b.ops.empty()) std::memcpy(ops, b.ops.data(), b.ops.size());

@@ -1376,7 +1376,7 @@ void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, c
     flush_segment(s, low, cfg, st, prof);
 }
 
-Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg)
+Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg, bool peel_swaps)
 {
     EV_REQUIRE(n >= fused::kMinTile, EV_E_ARG, "fused plans need at least 8 qubits");
     Lowering low(n);
@@ -1390,16 +1390,21 @@ Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg)
         }
     }
     low.flush_all();
+    std::vector<std::pair<int, int>> tail;
+    if (peel_swaps) tail = peel_swap_tail(low, cfg, n);
     const float budget = (cfg.t >= n) ? 1e30f : cfg.budget;
-    return plan(low.ops, low.pool, n, cfg.t, cfg.low, budget, cfg.coalesce);
+    Plan P = low.ops.empty() ? Plan{} : plan(low.ops, low.pool, n, cfg.t, cfg.low, budget, cfg.coalesce);
+    P.swap_tail = std::move(tail);
+    return P;
 }
 
 PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt)
 {
     const int n = int(c.num_qubits());
     const FusedConfig cfg = fused_config(n, opt.precision == EV_C64, opt);
-    const Plan P = plan_circuit(c, n, cfg);
+    const Plan P = plan_circuit(c, n, cfg, true);  // the plan a single run executes
     PlanBlob b;
+    b.swap_tail = P.swap_tail;
     auto bytes = [](const void *p, size_t len) {
         const unsigned char *c = static_cast<const unsigned char *>(p);
         return std::vector<unsigned char>(c, c + len);

@@ -2,6 +2,7 @@
 // tile passes (several gates per HBM sweep) and the pass kernel applies them.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "engine.hpp"
@@ -32,17 +33,24 @@ struct Plan {
     std::vector<fused::Stage> stages;
     std::vector<fused::DevOp> devops;
     std::vector<c128> coef;
+    // Disjoint qubit pairs whose SWAPs end the segment and run after the
+    // passes as one bit-permutation pass (empty unless the plan was made with
+    // the tail peeled off).
+    std::vector<std::pair<int, int>> swap_tail;
 };
 
 // Plans every gate of a measurement-free circuit for an n-qubit state.  When
 // the tile covers the whole state (t == n) the budget is lifted: the circuit
-// is one pass.
-Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg);
+// is one pass.  With peel_swaps, a trailing network of disjoint SWAPs wider
+// than a tile is taken out of the passes into swap_tail, as a single run
+// executes it; batched and served runs keep every gate in passes.
+Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg, bool peel_swaps = false);
 
 // Plans a measurement-free circuit without executing it (introspection).
 struct PlanBlob {
     std::vector<unsigned char> passes, stages, ops;
     std::vector<c128> coef;
+    std::vector<std::pair<int, int>> swap_tail;
     uint32_t np = 0, ns = 0, no = 0, t = 0;
 };
 PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt);

@@ -384,6 +384,10 @@ typedef struct {
     uint32_t num_passes, num_stages, num_ops, num_coef;
     uint32_t pass_bytes, stage_bytes, op_bytes;
     uint32_t tile_qubits;
+    /* a trailing network of disjoint SWAPs that the run executes after the
+       passes as one bit-permutation pass: qubit pairs (swap_a[i], swap_b[i]) */
+    uint32_t num_swap_pairs;
+    uint32_t swap_a[32], swap_b[32];
 } ev_plan_info;
 int ev_debug_plan(const ev_circuit *c, const ev_run_options *opt, void *passes, void *stages, void *ops,
                   ev_c128 *coef, ev_plan_info *info);

@@ -36,7 +36,10 @@ PHASE_ALL = 4
 MAX_PHASE_TERMS = 8
 
 
-def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low_qubits=0):
+def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low_qubits=0, with_tail=False):
+    """The plan a single run executes: passes, stages, ops, coefficients and
+    the tile width, and with with_tail the qubit pairs of the trailing SWAP
+    network it runs as one permutation pass after the passes."""
     from everett_amd.backend import run_options
 
     opt = run_options(precision, False, tile_qubits, max_gates_per_pass, low_qubits=low_qubits)
@@ -51,7 +54,10 @@ def get_plan(circuit, precision="c128", tile_qubits=0, max_gates_per_pass=0, low
     coef = np.zeros(max(info.num_coef, 1), dtype=np.complex128)
     check(_ffi.lib().ev_debug_plan(circuit.handle, C.byref(opt), _ffi.ptr(passes), _ffi.ptr(stages), _ffi.ptr(ops),
                                    _ffi.ptr(coef), C.byref(info)))
-    return passes, stages, ops[: info.num_ops], coef[: info.num_coef], info.tile_qubits
+    out = (passes, stages, ops[: info.num_ops], coef[: info.num_coef], info.tile_qubits)
+    if with_tail:
+        out += ([(int(info.swap_a[k]), int(info.swap_b[k])) for k in range(info.num_swap_pairs)],)
+    return out
 
 
 def swz(l, a, b):
@@ -253,8 +259,10 @@ def _lane(st, tid, t, base):
     return lb, gfix
 
 
-def emulate(psi, n, passes, stages, ops, coef):
-    """Runs the plan on psi (complex128, length 2^n) in place."""
+def emulate(psi, n, passes, stages, ops, coef, swap_tail=()):
+    """Runs the plan on psi (complex128, length 2^n) in place, then the SWAP
+    tail as the permutation pass does (amplitude x trades places with pi(x),
+    pi exchanging the bits of every pair)."""
     for P in passes:
         t = int(P["t"])
         W = 1 << (t - 4)
@@ -292,4 +300,11 @@ def emulate(psi, n, passes, stages, ops, coef):
                     gidx = (gfix[:, None] | st["goff"][None, :].astype(np.uint64)).astype(np.int64)
                     assert set(gidx.ravel().tolist()) == tile_idx, "scatter must cover the gathered tile"
                     psi[gidx] = v
+    if swap_tail:
+        x = np.arange(1 << n, dtype=np.int64)
+        y = x.copy()
+        for a, b in swap_tail:
+            d = ((x >> a) ^ (x >> b)) & 1
+            y ^= (d << a) | (d << b)
+        psi[:] = psi[y]
     return psi

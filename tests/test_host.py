@@ -218,7 +218,7 @@ def test_low_qubit_option_in_plans(low):
     from tests.fused_emulator import check_plan, emulate, get_plan
 
     c = ev.workloads.config1(10, 5)
-    p, s, o, cf, t = get_plan(c, tile_qubits=8, low_qubits=low)
+    p, s, o, cf, t, tail = get_plan(c, tile_qubits=8, low_qubits=low, with_tail=True)
     check_plan(10, p, s, o, cf)
     for P in p:
         assert list(P["q"][:low]) == list(range(low))
@@ -226,4 +226,4 @@ def test_low_qubit_option_in_plans(low):
     psi = rng.standard_normal(1 << 10) + 1j * rng.standard_normal(1 << 10)
     want = psi.copy()
     oracle.sv_apply_ops(c.ops(), want)
-    assert np.max(np.abs(emulate(psi, 10, p, s, o, cf) - want)) < 1e-12
+    assert np.max(np.abs(emulate(psi, 10, p, s, o, cf, tail) - want)) < 1e-12

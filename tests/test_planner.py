@@ -81,25 +81,45 @@ CASES = [
 def test_plan_emulation_matches_oracle(name, make, opts):
     c = make()
     n = c.num_qubits()
-    passes, stages, ops, coef, t = get_plan(c, **opts)
+    passes, stages, ops, coef, t, tail = get_plan(c, with_tail=True, **opts)
     check_plan(n, passes, stages, ops, coef, coalesce=4 if opts.get("precision") == "c64" else 3)
     rng = np.random.default_rng(n)
     psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
     psi /= np.linalg.norm(psi)
     want = psi.copy()
     oracle.sv_apply_ops(c.ops(), want)
-    got = emulate(psi.copy(), n, passes, stages, ops, coef)
+    got = emulate(psi.copy(), n, passes, stages, ops, coef, tail)
     assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("n,tile", [(16, 10), (18, 9), (20, 12)])
+def test_qft_swap_tail_is_peeled_and_emulates(n, tile):
+    # the QFT's bit reversal (n/2 disjoint SWAPs wider than a tile) leaves the passes and runs as one
+    # permutation after them, in the plan a single run executes; the emulated plan equals the oracle
+    c = ev.workloads.config1(n, 2)
+    passes, stages, ops, coef, t, tail = get_plan(c, tile_qubits=tile, with_tail=True)
+    assert len(tail) >= 2 and all(min(a, b) >= 3 for a, b in tail)
+    qs = [q for pr in tail for q in pr]
+    assert len(set(qs)) == len(qs)
+    check_plan(n, passes, stages, ops, coef)
+    rng = np.random.default_rng(n)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    want = psi.copy()
+    oracle.sv_apply_ops(c.ops(), want)
+    assert np.max(np.abs(emulate(psi.copy(), n, passes, stages, ops, coef, tail) - want)) < 1e-12
 
 
 def test_plan_structure_at_30_qubits():
     c = ev.workloads.config1()
-    passes, stages, ops, coef, t = get_plan(c)
+    passes, stages, ops, coef, t, tail = get_plan(c, with_tail=True)
     check_plan(30, passes, stages, ops, coef)
     tuned = ev.tune_get(30, "c128")
     assert t == (tuned["tile_qubits"] if tuned else 12)
     # fusion must beat one sweep per gate by a wide margin
     assert len(passes) < len(c) / 20
+    # the QFT's bit reversal above the 3 coalesced qubits is one permutation pass after the fused ones
+    assert sorted(tail) == sorted((29 - i, i) for i in range(3, 15))
 
 
 def test_plan_structure_c64_30_qubits():
