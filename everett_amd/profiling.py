@@ -126,6 +126,35 @@ def traffic(args) -> int:
     return 0
 
 
+def positions(args) -> int:
+    """Average duration and rate of one kernel's launches grouped by position:
+    launch i (in start order, the first `limit`) belongs to position
+    (i // repeat) % count, e.g. the one-qubit dense launches of a brick-wall
+    step, which come `repeat` per qubit in qubit order."""
+    con = sqlite3.connect(str(args.db))
+    rows = con.execute("SELECT name, start, end FROM kernels ORDER BY start").fetchall()
+    con.close()
+    d = [(e - s) * 1e-3 for n, s, e in rows if args.kernel in n][: args.limit]
+    per: dict[int, list[float]] = {}
+    for i, t in enumerate(d):
+        per.setdefault((i // args.repeat) % args.count, []).append(t)
+    lines = [f"{args.title} ({len(d)} launches of {args.kernel}, {args.bytes:.6g} B each; "
+             f"made by: python everett_amd/profiling.py positions --db DB --kernel {args.kernel} "
+             f"--repeat {args.repeat} --count {args.count} --limit {args.limit} --bytes {args.bytes:.0f})",
+             f"{'pos':>4} {'avg us':>9} {'GB/s':>7} {'of 8 TB/s':>10}"]
+    for k in sorted(per):
+        m = statistics.fmean(per[k])
+        lines.append(f"{k:>4} {m:>9.1f} {args.bytes / (m * 1e-6) / 1e9:>7.0f} {100 * args.bytes / (m * 1e-6) / 8e12:>9.1f}%")
+    m = statistics.fmean(d)
+    lines.append(f"{'all':>4} {m:>9.1f} {args.bytes / (m * 1e-6) / 1e9:>7.0f} {100 * args.bytes / (m * 1e-6) / 8e12:>9.1f}%")
+    text = "\n".join(lines) + "\n"
+    if args.append:
+        with open(args.append, "a") as f:
+            f.write("\n" + text)
+    print(text, end="")
+    return 0
+
+
 SQ_COUNTERS = ("SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAIT_INST_ANY",
                "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT")
 
@@ -203,8 +232,17 @@ def main(argv=None) -> int:
     q.add_argument("--kernel", required=True, help="substring of the kernel name")
     q.add_argument("--title", default="SQ counters per dispatch")
     q.add_argument("--out", required=True)
+    o = sub.add_parser("positions")
+    o.add_argument("--db", required=True)
+    o.add_argument("--kernel", required=True)
+    o.add_argument("--repeat", type=int, default=1)
+    o.add_argument("--count", type=int, required=True)
+    o.add_argument("--limit", type=int, default=1 << 30)
+    o.add_argument("--bytes", type=float, required=True)
+    o.add_argument("--title", default="launches by position")
+    o.add_argument("--append", default=None, help="also append the table to this file")
     args = ap.parse_args(argv)
-    return {"trace": trace, "traffic": traffic, "sq": sq}[args.cmd](args)
+    return {"trace": trace, "traffic": traffic, "sq": sq, "positions": positions}[args.cmd](args)
 
 
 if __name__ == "__main__":
