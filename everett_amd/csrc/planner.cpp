@@ -22,6 +22,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -258,6 +259,31 @@ class StageFuser {
 
     void add(int id) { ids_.push_back(id); }
 
+    // CNOTs and SWAPs between register qubits that the block choice leaves
+    // bare (neither opening a block nor merging into the block of the same
+    // pair just before them), in stage order.
+    //
+    // For this question one-qubit runs that end the stage on their qubit
+    // (no later pair op takes them) count for nothing: when such a CNOT
+    // moves to the exchange, they wait for the next stage, where they open
+    // the next layer's blocks instead.
+    std::vector<int> bare_perms()
+    {
+        choose_centers(true);
+        std::vector<int> out;
+        for (int id : ids_) {
+            const GOp &op = g_[id];
+            if ((op.kind != G_CX && op.kind != G_SWAP) || center_.count(id)) continue;
+            Mat4 m4;
+            int a, b;
+            if (!as_pair(op, &a, &b, &m4)) continue;
+            auto it = same_pair_.find(id);
+            if (it != same_pair_.end() && center_.count(it->second)) continue;  // it multiplies into that block
+            out.push_back(id);
+        }
+        return out;
+    }
+
     // The fused stage as planner ops; new dense 4x4 and 2x2 coefficients go to
     // pool.  A pair block becomes one dense 4x4 when its member ops cost at
     // least as much; otherwise its members run as they are.
@@ -341,7 +367,7 @@ class StageFuser {
     // weighted cover: a candidate is chosen while the runs it would newly
     // absorb cost more than the block adds over running the op bare (dense
     // 4x4s are always blocks: they cost the same either way).
-    void choose_centers()
+    void choose_centers(bool free_tails = false)
     {
         struct Edge {
             int prev, next;
@@ -382,6 +408,9 @@ class StageFuser {
             if (as_pair(op, &a, &b, &m4)) {
                 cands.push_back(id);
                 inc[id];
+                auto la = last.find(a), lb = last.find(b);
+                if (la != last.end() && lb != last.end() && la->second >= 0 && la->second == lb->second)
+                    same_pair_[id] = la->second;  // nothing but one-qubit runs since the previous op on this pair
                 for (int q : {a, b}) {
                     close_run(q, id);
                     last[q] = id;
@@ -407,7 +436,7 @@ class StageFuser {
                 if (center_.count(c)) continue;
                 float w = 0;
                 for (int e : inc[c])
-                    if (!edges[size_t(e)].covered) w += edges[size_t(e)].w;
+                    if (!edges[size_t(e)].covered && !(free_tails && edges[size_t(e)].next < 0)) w += edges[size_t(e)].w;
                 const float gain = w - (block - fused_cost(g_[c].kind));
                 if (gain > best_gain) {
                     best_gain = gain;
@@ -591,7 +620,8 @@ class StageFuser {
     const std::vector<c128> &pool_;
     uint64_t regq_;
     std::vector<int> ids_;
-    std::set<int> center_;  // pair ops that open a block
+    std::set<int> center_;          // pair ops that open a block
+    std::map<int, int> same_pair_;  // pair op -> the previous pair op on the same two qubits
     std::vector<Block> blocks_;
     std::map<int, int> open_;
 };
@@ -858,7 +888,7 @@ int bank_degree(const uint32_t *v, int nv, int bits, uint32_t a, uint32_t b)
 // slots it read and needs no barrier between its read and its write.  The
 // score is the bank degree of the writes and the reads of every exchange,
 // for 32 lanes per cycle (8-byte slots), then for 64 (4-byte slots).
-void pass_swizzle(Pass &ps, Stage *st)
+void pass_swizzle(Pass &ps, Stage *st, bool wide64)
 {
     const int ns = ps.nstages, np = ps.t - kRegBits;
     uint32_t best_a = 6, best_b = fused::kSwzNone;
@@ -878,14 +908,17 @@ void pass_swizzle(Pass &ps, Stage *st)
         // (a, b) with 2 <= a < b <= 12.
         static constexpr uint32_t cand[][2] = {{3, 4}, {2, 4}, {3, 5}, {2, 5}, {2, 3}, {4, 7},
                                                {4, 5}, {3, 6}, {2, 6}, {4, 6}, {5, 6}, {6, fused::kSwzNone}};
+        // the lanes that share an LDS cycle: 32 with 8-byte slots (complex128),
+        // all 64 with 4-byte slots (complex64); their bank degree counts first
+        const int major = wide64 ? 6 : 5, minor = wide64 ? 5 : 6;
         long best = -1;
         const long ideal = 2L * (ns - 1) * 1000 + 2L * (ns - 1);
         for (const auto &c : cand) {
             long sc = 0;
             for (size_t k = 0; k < wr.size(); ++k)
-                sc += 1000L * (bank_degree(wr[k].data(), np, 5, c[0], c[1]) +
-                               bank_degree(rd[k].data(), np, 5, c[0], c[1])) +
-                      bank_degree(wr[k].data(), np, 6, c[0], c[1]) + bank_degree(rd[k].data(), np, 6, c[0], c[1]);
+                sc += 1000L * (bank_degree(wr[k].data(), np, major, c[0], c[1]) +
+                               bank_degree(rd[k].data(), np, major, c[0], c[1])) +
+                      bank_degree(wr[k].data(), np, minor, c[0], c[1]) + bank_degree(rd[k].data(), np, minor, c[0], c[1]);
             if (best < 0 || sc < best) {
                 best = sc;
                 best_a = c[0];
@@ -1012,11 +1045,14 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             std::vector<uint32_t> lin;
             std::vector<std::pair<int, uint32_t>> aff;
         };
-        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake, const Relabel *relabel) {
-            uint64_t regq = 0;  // global qubits held in registers
+        auto regq_of = [&](uint32_t S) {  // global qubits held in registers
+            uint64_t regq = 0;
             for (int l = 0; l < t; ++l)
                 if ((S >> l) & 1) regq |= uint64_t(1) << ps.q[l];
-            StageFuser fuser(g, pool, regq);
+            return regq;
+        };
+        auto emit_stage = [&](uint32_t S, const std::vector<int> &stake, const Relabel *relabel) {
+            StageFuser fuser(g, pool, regq_of(S));
             for (int id : stake) fuser.add(id);
             const std::vector<GOp> fused = fuser.finish(pool);
             Stage st{};
@@ -1100,6 +1136,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             std::vector<int> stake, sdefer;
             a_out = ident;
             bool out_pending = false;
+            uint64_t out_touched = 0;  // qubits of the relabellings composed into a_out
             for (int id : srem) {
                 const GOp &op = g[id];
                 const uint64_t tq = op.touched();
@@ -1146,6 +1183,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                         std::swap(a_out.lin[local(op.q0)], a_out.lin[local(op.q1)]);
                     }
                     out_pending = true;
+                    out_touched |= tq;
                     // later ops on its qubits wait for the next stage (no op of
                     // the pass targets an outside control)
                     sblocked |= affine ? (uint64_t(1) << op.q0) : tq;
@@ -1161,6 +1199,55 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                     sdefer.push_back(id);
                 }
             }
+            // A CNOT or SWAP between register qubits that would run bare costs a
+            // register shuffle; the exchange after the stage applies it for free
+            // when the stage's later ops on its qubits wait for the next stage.
+            // In a brick-wall window this leaves each block the rotations on both
+            // sides of its own CNOT (four per 4x4) instead of sharing them with
+            // the CNOTs of the next layer.  A move is taken only when no
+            // relabelling already composed for this exchange (a later op) touches
+            // its qubits or the qubits of the ops it sends on, so the composed
+            // order stays the program order.
+            for (;;) {
+                StageFuser probe(g, pool, regq_of(S));
+                for (int id : stake) probe.add(id);
+                int chosen = -1;
+                std::vector<int> keep, moved;
+                for (int c : probe.bare_perms()) {
+                    uint64_t blk = g[c].touched();
+                    keep.clear();
+                    moved.clear();
+                    bool after = false;
+                    for (int id : stake) {
+                        if (id == c) {
+                            after = true;
+                        } else if (after && (g[id].touched() & blk)) {
+                            blk |= g[id].touched();
+                            moved.push_back(id);
+                        } else {
+                            keep.push_back(id);
+                        }
+                    }
+                    if (!(blk & out_touched)) {
+                        chosen = c;
+                        break;
+                    }
+                }
+                if (chosen < 0) break;
+                const GOp &op = g[chosen];
+                if (op.kind == G_CX) {
+                    const int lc = local(__builtin_ctzll(op.cmask)), lt = local(op.q0);
+                    a_out.lin[lc] ^= a_out.lin[lt];
+                } else {
+                    std::swap(a_out.lin[local(op.q0)], a_out.lin[local(op.q1)]);
+                }
+                out_pending = true;
+                out_touched |= op.touched();
+                stake.swap(keep);
+                std::vector<int> merged(sdefer.size() + moved.size());
+                std::merge(sdefer.begin(), sdefer.end(), moved.begin(), moved.end(), merged.begin());
+                sdefer.swap(merged);  // ids are in program order
+            }
             srem.swap(sdefer);
             S = pad_registers(S, t, lowc, first);
             emit_stage(S, stake, in_pending ? &a_in : nullptr);
@@ -1175,7 +1262,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         if ((lastS & lowc) || in_pending) emit_stage(pad_registers(0, t, lowc, true), {}, in_pending ? &a_in : nullptr);
         ps.nstages = int(P.stages.size()) - ps.stage_begin;
         P.stages.back().gneed = 1;  // the last stage scatters to global addresses
-        pass_swizzle(ps, P.stages.data() + ps.stage_begin);
+        pass_swizzle(ps, P.stages.data() + ps.stage_begin, coalesce >= 4);
         P.passes.push_back(ps);
     }
     return P;
