@@ -15,8 +15,10 @@
 // the amplitudes in HBM.
 struct ev_sv {
     int n = 0;
-    void *plan_buf = nullptr;       // device copy of the fused pass descriptors
-    size_t plan_cap = 0;
+    void *plan_buf = nullptr;       // device ring of fused pass descriptor blocks
+    void *plan_host = nullptr;      // its pinned host staging (same offsets)
+    size_t plan_cap = 0;            // bytes in each
+    size_t plan_off = 0;            // next free offset in the ring
     ev::kern::Prec prec = ev::kern::C128;
     void *psi = nullptr;            // 2^n amplitudes
     hipStream_t stream = nullptr;   // all work on this handle is ordered here

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <set>
 #include <vector>
@@ -945,8 +946,13 @@ void pass_swizzle(Pass &ps, Stage *st, bool wide64)
 // c64): the first and last stage of every pass keep their register bits off
 // those positions (where the tile holds the state's lowest qubits), so the
 // kernel's global gather and scatter move whole runs.
+//
+// `on_pass`, when given, is called with the plan so far each time a pass is
+// complete (later passes never change it), so a caller can launch pass p
+// while pass p + 1 is planned.
+using PassSink = std::function<void(const Plan &, size_t)>;
 Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, int t, int low, float budget,
-          int coalesce)
+          int coalesce, const PassSink &on_pass = {})
 {
     Plan P;
     std::vector<c128> pool = pool_in;  // grows with fused coefficients
@@ -1264,6 +1270,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         P.stages.back().gneed = 1;  // the last stage scatters to global addresses
         pass_swizzle(ps, P.stages.data() + ps.stage_begin, coalesce >= 4);
         P.passes.push_back(ps);
+        if (on_pass) on_pass(P, P.passes.size() - 1);
     }
     return P;
 }
@@ -1303,62 +1310,103 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
 
 namespace {
 
-// Uploads a plan and launches its passes on s's stream.
-void run_plan(ev_sv *s, const Plan &P, RunStats &st, Profiler &prof)
-{
-    if (P.passes.empty()) return;
-    const bool c64 = s->prec == kern::C64;
-    const size_t ab = kern::amp_bytes(s->prec);
-    const size_t o_pass = 0;
-    const size_t o_stage = align256(P.passes.size() * sizeof(Pass));
-    const size_t o_ops = o_stage + align256(P.stages.size() * sizeof(Stage));
-    const size_t o_coef = o_ops + align256(std::max<size_t>(1, P.devops.size()) * sizeof(DevOp));
-    const size_t total = o_coef + align256(std::max<size_t>(1, P.coef.size()) * ab);
-    std::vector<unsigned char> host(total, 0);
-    std::memcpy(host.data() + o_pass, P.passes.data(), P.passes.size() * sizeof(Pass));  // P.passes is not empty
-    if (!P.stages.empty()) std::memcpy(host.data() + o_stage, P.stages.data(), P.stages.size() * sizeof(Stage));
-    if (!P.devops.empty()) std::memcpy(host.data() + o_ops, P.devops.data(), P.devops.size() * sizeof(DevOp));
-    if (c64) fused::inline_coef_to_c64(reinterpret_cast<DevOp *>(host.data() + o_ops), P.devops.size());
-    if (!c64) {
-        if (!P.coef.empty()) std::memcpy(host.data() + o_coef, P.coef.data(), P.coef.size() * sizeof(c128));
-    } else {
-        float *f = reinterpret_cast<float *>(host.data() + o_coef);
-        for (size_t i = 0; i < P.coef.size(); ++i) {
-            f[2 * i] = float(P.coef[i].re);
-            f[2 * i + 1] = float(P.coef[i].im);
-        }
-    }
-    if (s->plan_cap < total) {
-        sv_sync(s);
-        if (s->plan_buf) EV_HIP(hipFree(s->plan_buf));
-        s->plan_buf = nullptr;
-        s->plan_cap = 0;
-        EV_HIP(hipMalloc(&s->plan_buf, total));
-        s->plan_cap = total;
-    }
-    unsigned char *dev = static_cast<unsigned char *>(s->plan_buf);
-    EV_HIP(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, s->stream));
-    EV_HIP(hipStreamSynchronize(s->stream));  // the host staging buffer is released on return
-    const Pass *dp = reinterpret_cast<const Pass *>(dev + o_pass);
-    const Stage *ds = reinterpret_cast<const Stage *>(dev + o_stage);
-    const DevOp *dops = reinterpret_cast<const DevOp *>(dev + o_ops);
-    const void *dcoef = dev + o_coef;
-    for (size_t p = 0; p < P.passes.size(); ++p) {
-        prof.begin();
-        fused::launch_pass(s->psi, c64, s->n, P.passes[p].t, dp + p, ds, dops, dcoef, s->stream);
-        prof.end("fused_pass", full_pass_bytes(s->n, ab));
+// Streams a plan's passes to the device as the planner completes them: each
+// pass goes up as a self-contained block (its Pass, stages, ops and
+// coefficients, indices rebased to the block) through the handle's pinned
+// staging ring, on the handle's stream, and is launched at once, so the
+// device runs pass p while the host plans pass p + 1.  Blocks take
+// consecutive ring offsets; when the next one does not fit, the stream is
+// drained (every earlier copy has landed) and the ring starts over.
+class PassStreamer {
+  public:
+    PassStreamer(ev_sv *s, RunStats &st, Profiler &prof) : s_(s), st_(st), prof_(prof) {}
+
+    void operator()(const Plan &P, size_t p)
+    {
+        const bool c64 = s_->prec == kern::C64;
+        const size_t ab = kern::amp_bytes(s_->prec);
         const Pass &ps = P.passes[p];
-        double fpa = 0;
+        const Stage *stg = P.stages.data() + ps.stage_begin;
+        const int op0 = stg[0].op_begin, op1 = stg[ps.nstages - 1].op_end;
+        const size_t coef0 = coef_done_, coef1 = P.coef.size();
+        coef_done_ = coef1;
+        const size_t nops = size_t(op1 - op0), ncoef = coef1 - coef0;
+        const size_t o_stage = align256(sizeof(Pass));
+        const size_t o_ops = o_stage + align256(size_t(ps.nstages) * sizeof(Stage));
+        const size_t o_coef = o_ops + align256(std::max<size_t>(1, nops) * sizeof(DevOp));
+        const size_t total = o_coef + align256(std::max<size_t>(1, ncoef) * ab);
+        unsigned char *host = nullptr, *dev = nullptr;
+        reserve(total, &host, &dev);
+
+        Pass *hp = reinterpret_cast<Pass *>(host);
+        *hp = ps;
+        hp->stage_begin = 0;
+        Stage *hs = reinterpret_cast<Stage *>(host + o_stage);
         for (int k = 0; k < ps.nstages; ++k) {
-            const Stage &sg = P.stages[size_t(ps.stage_begin + k)];
-            for (int o = sg.op_begin; o < sg.op_end; ++o) fpa += op_flops_per_amp(P.devops[size_t(o)]);
+            hs[k] = stg[k];
+            hs[k].op_begin -= op0;
+            hs[k].op_end -= op0;
         }
-        st.pass_flops += fpa * std::ldexp(1.0, s->n);
-        ++st.passes;
-        ++st.pass_launches;
-        ++st.launches;
+        DevOp *ho = reinterpret_cast<DevOp *>(host + o_ops);
+        double fpa = 0;
+        for (size_t i = 0; i < nops; ++i) {
+            const DevOp &d = P.devops[size_t(op0) + i];
+            fpa += op_flops_per_amp(d);
+            ho[i] = d;
+            ho[i].coef = d.coef - uint32_t(coef0);  // every op of the pass was lowered after coef0
+        }
+        if (c64) fused::inline_coef_to_c64(ho, nops);
+        if (!c64) {
+            if (ncoef) std::memcpy(host + o_coef, P.coef.data() + coef0, ncoef * sizeof(c128));
+        } else {
+            float *f = reinterpret_cast<float *>(host + o_coef);
+            for (size_t i = 0; i < ncoef; ++i) {
+                f[2 * i] = float(P.coef[coef0 + i].re);
+                f[2 * i + 1] = float(P.coef[coef0 + i].im);
+            }
+        }
+        EV_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s_->stream));
+        prof_.begin();
+        fused::launch_pass(s_->psi, c64, s_->n, ps.t, reinterpret_cast<const Pass *>(dev),
+                           reinterpret_cast<const Stage *>(dev + o_stage), reinterpret_cast<const DevOp *>(dev + o_ops),
+                           dev + o_coef, s_->stream);
+        prof_.end("fused_pass", full_pass_bytes(s_->n, ab));
+        st_.pass_flops += fpa * std::ldexp(1.0, s_->n);
+        ++st_.passes;
+        ++st_.pass_launches;
+        ++st_.launches;
     }
-}
+
+  private:
+    // Ring space for a block of `bytes`: grows the ring (after draining the
+    // stream) for a block larger than it, starts over when it is full.
+    void reserve(size_t bytes, unsigned char **host, unsigned char **dev)
+    {
+        ev_sv *s = s_;
+        if (bytes > s->plan_cap) {
+            sv_sync(s);
+            if (s->plan_buf) EV_HIP(hipFree(s->plan_buf));
+            if (s->plan_host) EV_HIP(hipHostFree(s->plan_host));
+            s->plan_buf = s->plan_host = nullptr;
+            s->plan_cap = s->plan_off = 0;
+            const size_t cap = std::max(bytes * 2, size_t(s->n >= 20 ? 4 : 1) << 20);
+            EV_HIP(hipMalloc(&s->plan_buf, cap));
+            EV_HIP(hipHostMalloc(&s->plan_host, cap, hipHostMallocDefault));
+            s->plan_cap = cap;
+        } else if (s->plan_off + bytes > s->plan_cap) {
+            sv_sync(s);  // the ring's earlier copies have all landed
+            s->plan_off = 0;
+        }
+        *host = static_cast<unsigned char *>(s->plan_host) + s->plan_off;
+        *dev = static_cast<unsigned char *>(s->plan_buf) + s->plan_off;
+        s->plan_off += bytes;
+    }
+
+    ev_sv *s_;
+    RunStats &st_;
+    Profiler &prof_;
+    size_t coef_done_ = 0;
+};
 
 double now_ms()
 {
@@ -1402,9 +1450,14 @@ void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st
     const std::vector<std::pair<int, int>> tail = peel_swap_tail(low, cfg, s->n);
     const float budget = (cfg.t >= s->n) ? 1e30f : cfg.budget;  // one tile holds the state: one pass
     if (!low.ops.empty()) {
-        const Plan P = plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce);
-        st.plan_ms += now_ms() - t0;
-        run_plan(s, P, st, prof);
+        PassStreamer launch(s, st, prof);
+        double in_sink = 0;  // host time spent uploading and launching, not planning
+        plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce, [&](const Plan &P, size_t p) {
+            const double a = now_ms();
+            launch(P, p);
+            in_sink += now_ms() - a;
+        });
+        st.plan_ms += now_ms() - t0 - in_sink;
     }
     if (!tail.empty()) {
         std::vector<int> a, b;

@@ -60,6 +60,7 @@ void sv_destroy(ev_sv *s)
     if (s->psi) (void)hipFree(s->psi);
     if (s->d_red) (void)hipFree(s->d_red);
     if (s->plan_buf) (void)hipFree(s->plan_buf);
+    if (s->plan_host) (void)hipHostFree(s->plan_host);
     if (s->h_red) (void)hipHostFree(s->h_red);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
