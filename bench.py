@@ -129,14 +129,14 @@ def oracle_fidelity(circuits, idxs, state_of) -> dict:
             "max_abs": float(max(mabs)) if mabs else None, "oracle_s": time.perf_counter() - t0}
 
 
-def run_sv_workload(circuit, n, precision, steps, warmup, plain):
+def run_sv_workload(circuit, n, precision, steps, warmup, plain, mfma_u2=False):
     import everett_amd as ev
 
     t = time.perf_counter()
     state = ev.State.zero(n, precision)
     alloc_ms = (time.perf_counter() - t) * 1e3
     for _ in range(warmup):
-        ev.run_into(state, circuit, 0, plain=plain)
+        ev.run_into(state, circuit, 0, plain=plain, mfma_u2=mfma_u2)
     state.norm_sqr()  # synchronises the handle's stream
     agg: dict = {}
     dev_ms = 0.0
@@ -144,7 +144,7 @@ def run_sv_workload(circuit, n, precision, steps, warmup, plain):
     pass_flops = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
-        ev.run_into(state, circuit, 0, plain=plain, profile=True)
+        ev.run_into(state, circuit, 0, plain=plain, profile=True, mfma_u2=mfma_u2)
         rs = ev.last_run_stats()
         dev_ms += rs.device_ms
         passes += rs.passes
@@ -208,7 +208,7 @@ def compute_roofline(agg, pass_flops, precision):
 
 
 def bench_sv(name, circuit, n, desc, args, steps, warmup):
-    r = run_sv_workload(circuit, n, args.precision, steps, warmup, args.plain)
+    r = run_sv_workload(circuit, n, args.precision, steps, warmup, args.plain, getattr(args, "mfma_u2", False))
     gates = len(circuit)
     step_bytes = sum(a["bytes"] for a in r["agg"].values()) / steps
     return {
@@ -439,6 +439,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=None, help="override the c1 qubit count (profiling runs)")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     ap.add_argument("--plain", action="store_true", help="one launch per gate instead of fused passes")
+    ap.add_argument("--mfma-u2", action="store_true",
+                    help="complex64: dense 4x4 gates on the matrix cores instead of VALU FMA chains")
     ap.add_argument("--no-autotune", action="store_true", help="never tune; sizes without a table entry use defaults")
     ap.add_argument("--retune", action="store_true", help="tune the c1 size even when the table has an entry")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -508,6 +510,15 @@ def main() -> int:
         extra["c1_c64"] = {k: r[k] for k in ("workload", "gates", "gates_per_s", "ms_per_step", "device_ms_per_step",
                                              "passes_per_step", "roofline", "compute_roofline", "kernels")}
         log(f"c1 c64: {r['gates_per_s']:.0f} gates/s")
+        # the same passes with the dense 4x4 gates on the matrix cores (v_mfma_f32_4x4x1_16b_f32)
+        m = bench_sv("c1_c64", c1, n1, "", argparse.Namespace(**dict(vars(args), precision="c64", mfma_u2=True)),
+                     steps, warmup)
+        extra["c1_c64"]["mfma_u2"] = {
+            "gates_per_s": m["gates_per_s"], "ms_per_step": m["ms_per_step"],
+            "device_ms_per_step": m["device_ms_per_step"], "vs_valu": m["gates_per_s"] / r["gates_per_s"],
+            "note": "dense 4x4 gates as 64 v_mfma_f32_4x4x1_16b_f32 per op per wave (twice the unpacked VALU FP32 "
+                    "rate on the op alone); the kernel then holds 116 VGPRs (4 waves per SIMD instead of 6)"}
+        log(f"c1 c64 with matrix-core 4x4 gates: {m['gates_per_s']:.0f} gates/s")
     if wl in ("all", "c0"):
         c0 = ev.workloads.config0(20, 20)
         r = bench_sv("c0", c0, 20, "20-qubit brick-wall depth 20 (seed 20)", args, 10, 2)

@@ -104,6 +104,7 @@ class EvRunOptions(C.Structure):
         ("use_graph", C.c_uint32),
         ("renormalize", C.c_uint32),
         ("low_qubits", C.c_uint32),
+        ("mfma_u2", C.c_uint32),
     ]
 
 

@@ -14,9 +14,10 @@ from .state import State, precision_code
 
 
 def run_options(precision="c128", plain: bool = False, tile_qubits: int = 0, max_gates_per_pass: int = 0,
-                profile: bool = False, low_qubits: int = 0):
+                profile: bool = False, low_qubits: int = 0, mfma_u2: bool = False):
     o = _ffi.EvRunOptions()
     o.low_qubits = low_qubits
+    o.mfma_u2 = 1 if mfma_u2 else 0
     o.precision = precision_code(precision)
     o.plain = 1 if plain else 0
     o.tile_qubits = tile_qubits
@@ -96,8 +97,8 @@ class StateVectorBackend:
 
     @staticmethod
     def run_seeded(circuit: Circuit, seed: int, precision="c128", plain: bool = False, tile_qubits: int = 0,
-                   max_gates_per_pass: int = 0, low_qubits: int = 0) -> Execution:
-        opt = run_options(precision, plain, tile_qubits, max_gates_per_pass, low_qubits=low_qubits)
+                   max_gates_per_pass: int = 0, low_qubits: int = 0, mfma_u2: bool = False) -> Execution:
+        opt = run_options(precision, plain, tile_qubits, max_gates_per_pass, low_qubits=low_qubits, mfma_u2=mfma_u2)
         ncl = circuit.num_classical()
         bits = np.zeros(max(ncl, 1), dtype=np.uint8)
         h = C.c_void_p()

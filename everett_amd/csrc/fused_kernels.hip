@@ -189,6 +189,69 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(const R
     }
 }
 
+// Complex64 dense 4x4 on the matrix cores: v_mfma_f32_4x4x1_16b_f32 runs 16
+// independent 4x4 outer products, block b = lanes 4b..4b+3, where lane 4b+j
+// holds column j of B (one float) and receives column j of D (four floats),
+// and lane 4b+i holds row i of A.  With A = column k of the gate's real 8x8
+// form (lane 4b+i holding row i) and B = input component k of the lane's own
+// group, D accumulates the lane's own four output components: each lane
+// multiplies its own groups, no amplitude crosses lanes.  Eight rank-1 steps
+// per output half (re: Re M x re - Im M x im, im: Im M x re + Re M x im), 64
+// MFMAs per op per wave at twice the unpacked VALU's FP32 rate.
+typedef float EvF4 __attribute__((ext_vector_type(4)));
+
+// The lane's row of a complex64 gate (the A operands), loaded ahead of the op.
+struct U2Row {
+    float ar[4], ai[4];
+};
+
+template <class V, int BH, int BL>
+__device__ __forceinline__ void op_u2_mfma(const Regs<V> &in, Regs<V> &out, const U2Row &r)
+{
+    constexpr int kG = kRegs / 4;
+    int g[kG][4];
+    {
+        int n = 0;
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) {
+            if (j & ((1 << BH) | (1 << BL))) continue;
+            g[n][0] = j;
+            g[n][1] = j | (1 << BL);
+            g[n][2] = j | (1 << BH);
+            g[n][3] = j | (1 << BH) | (1 << BL);
+            ++n;
+        }
+    }
+    // k outer, groups inner: eight independent accumulation chains in flight
+    EvF4 ore[kG], oim[kG];
+#pragma unroll
+    for (int q = 0; q < kG; ++q) {
+        ore[q] = EvF4{0.f, 0.f, 0.f, 0.f};
+        oim[q] = EvF4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int q = 0; q < kG; ++q) {
+            ore[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(r.ar[k], in[g[q][k]].x, ore[q], 0, 0, 0);
+            oim[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(r.ai[k], in[g[q][k]].x, oim[q], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float an = -r.ai[k];
+#pragma unroll
+        for (int q = 0; q < kG; ++q) {
+            ore[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(an, in[g[q][k]].y, ore[q], 0, 0, 0);
+            oim[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(r.ar[k], in[g[q][k]].y, oim[q], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kG; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[g[q][i]] = mk<V>(ore[q][i], oim[q][i]);
+}
+
 template <class V, int BH, int BL> __device__ __forceinline__ void op_swap(const Regs<V> &in, Regs<V> &out)
 {
 #pragma unroll
@@ -267,9 +330,30 @@ template <class V> __device__ __forceinline__ void load_row(cptr<V> m, V *out)
     for (int k = 0; k < 16; ++k) out[k] = mk<V>(f[2 * k], f[2 * k + 1]);
 }
 
-template <class V>
+// M (complex64 only): dense 4x4 gates run on the matrix cores (op_u2_mfma)
+// instead of VALU fused multiply-add chains.
+template <class V, bool M> constexpr bool kMfmaU2 = M && std::is_same_v<decltype(V::x), float>;
+
+// The lane's row (lane mod 4) of a kU2 gate for op_u2_mfma; nothing otherwise.
+template <class V, bool M> __device__ __forceinline__ U2Row u2_row(const OpView<V> &op, cptr<V> coef)
+{
+    U2Row r{};
+    if constexpr (kMfmaU2<V, M>) {
+        if ((op.code & 0xff) == kU2) {
+            const cptr<float> m = (cptr<float>)(coef + op.coef + 4 * (threadIdx.x & 3u));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r.ar[k] = m[2 * k];
+                r.ai[k] = m[2 * k + 1];
+            }
+        }
+    }
+    return r;
+}
+
+template <class V, bool M>
 __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, Regs<V> &out, uint64_t gfix,
-                                       cptr<V> coef)
+                                       cptr<V> coef, const U2Row &row)
 {
     const uint32_t kind = op.code & 0xff, b0 = (op.code >> 8) & 0xff, b1 = (op.code >> 16) & 0xff;
     const uint32_t flags = op.code >> 24;
@@ -307,9 +391,13 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
         break;
     }
     case kU2: {
-        V m[16];
-        load_row<V>(coef + op.coef, m);
-        EV_CASE_PAIR(op_u2, in, out, m);
+        if constexpr (kMfmaU2<V, M>) {
+            EV_CASE_PAIR(op_u2_mfma, in, out, row);
+        } else {
+            V m[16];
+            load_row<V>(coef + op.coef, m);
+            EV_CASE_PAIR(op_u2, in, out, m);
+        }
         break;
     }
     case kPhase: {
@@ -417,7 +505,7 @@ __device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Re
 // descriptor loads are in flight while the current op computes); an odd
 // last op is followed by one copy back to v.  (Exchanging straight from w
 // instead keeps both arrays live across the exchange, which spills.)
-template <class V>
+template <class V, bool M>
 __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V> &w, uint64_t gfix, cptr<DevOp> ops,
                                               cptr<V> coef)
 {
@@ -425,11 +513,14 @@ __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V
     int o = ob;
     for (; o + 1 < oe; o += 2) {
         const OpView<V> a = fetch_op<V>(ops + o), b = fetch_op<V>(ops + o + 1);
-        run_op<V>(a, v, w, gfix, coef);
-        run_op<V>(b, w, v, gfix, coef);
+        // matrix-core gates: both rows are in flight before the first op runs
+        const U2Row ra = u2_row<V, M>(a, coef), rb = u2_row<V, M>(b, coef);
+        run_op<V, M>(a, v, w, gfix, coef, ra);
+        run_op<V, M>(b, w, v, gfix, coef, rb);
     }
     if (o < oe) {
-        run_op<V>(fetch_op<V>(ops + o), v, w, gfix, coef);
+        const OpView<V> a = fetch_op<V>(ops + o);
+        run_op<V, M>(a, v, w, gfix, coef, u2_row<V, M>(a, coef));
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) v[j] = w[j];
     }
@@ -517,7 +608,7 @@ __device__ __forceinline__ void exchange(Regs<Amp<R>> &v, void *lds, bool active
 // widest tile) run the register arithmetic on don't-care values but skip
 // every memory access, and keep every barrier; the op loop thus stays out of
 // divergent control flow and its descriptor reads stay scalar.
-template <class R>
+template <class R, bool M>
 __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v, void *lds)
 {
     using V = Amp<R>;
@@ -527,7 +618,7 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
     Lane ln = lane_of(tr.stages, tid, tr.base, tr.swz_a, tr.swz_b);
     for (int s = 0; s < nst; ++s) {
         const cptr<Stage> st = tr.stages + s;
-        run_stage_ops<V>(st, v, w, ln.gfix, tr.ops, tr.coef);
+        run_stage_ops<V, M>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
             const cptr<Stage> nx = st + 1;
             // the global index bits only where an op of the stage tests them or the stage scatters
@@ -561,32 +652,34 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
 // dispatcher keeps every CU full; a persistent grid that loops over tiles
 // measured 1.7x slower on MI355X (any CU slot it fails to place delays a
 // whole strip of tiles).
-template <class R>
+template <class R, bool M>
 __device__ __forceinline__ void one_tile(const TileRef<R> &tr, void *lds)
 {
     using V = Amp<R>;
     const uint32_t tid = threadIdx.x;
     Regs<V> v;
     load_tile<R>(tr, tid, v);  // threads beyond the tile read an in-tile alias (their bits above t-4 are ignored)
-    compute_tile<R>(tr, tid, v, lds);
+    compute_tile<R, M>(tr, tid, v, lds);
 }
 
 // Register budget: 168 VGPRs (3 waves per SIMD; the two c128 register
 // arrays alone take 128).  With the split exchange a tile of 2^t amplitudes
 // takes 2^t reals of LDS, so registers, not LDS, bound the resident waves.
+// (The complex64 kernel takes 77 VGPRs, 6 waves; with matrix-core gates,
+// whose row operands and accumulators live in VGPRs too, 116, 4 waves.)
 template <class R> constexpr int kWavesPerSimd = 3;
 
 // XCD-aware tile order: dispatch places workgroup b on XCD b mod 8, so XCD x
 // gets the contiguous tile range [x * tiles/8, (x+1) * tiles/8), and tiles
 // that neighbour in memory share an L2.
-template <class R>
+template <class R, bool M>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_pass(
     Amp<R> *__restrict__ psi, const Pass *__restrict__ pass, const Stage *__restrict__ stages,
     const DevOp *__restrict__ ops, const Amp<R> *__restrict__ coef, uint32_t per, uint32_t xcds)
 {
     extern __shared__ unsigned char lds_raw[];
     const uint64_t T = uint64_t(blockIdx.x % xcds) * per + blockIdx.x / xcds;
-    one_tile<R>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
+    one_tile<R, M>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
 }
 
 // Batch kernel: workgroup b runs linear tile b of the round, whose entry
@@ -605,12 +698,12 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
         if (ent[mid].tile_begin <= lin) lo = mid; else hi = mid - 1;
     }
     const cptr<BatchEntry> e = ent + lo;
-    one_tile<R>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
+    one_tile<R, false>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
                             as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin + e->tile_off),
                 lds_raw);
 }
 
-template <class R>
+template <class R, bool M>
 void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops, const void *coef,
                    hipStream_t st)
 {
@@ -618,7 +711,7 @@ void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stage
     const unsigned block = 1u << (t - kRegBits);  // <= kMaxThreads = the kernel's launch bound
     const size_t lds = (size_t(1) << t) * kLdsSlotBytes<R>;
     const uint32_t xcds = (tiles % 8 == 0) ? 8u : 1u;
-    hipLaunchKernelGGL(k_fused_pass<R>, dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
+    hipLaunchKernelGGL((k_fused_pass<R, M>), dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
                        stages, ops, static_cast<const Amp<R> *>(coef), uint32_t(tiles / xcds), xcds);
 }
 
@@ -633,12 +726,14 @@ void launch_batch_t(const BatchEntry *entries, int nentries, uint64_t total_tile
 }  // namespace
 
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
-                 const void *coef, hipStream_t st)
+                 const void *coef, bool mfma_u2, hipStream_t st)
 {
     EV_REQUIRE(t >= kMinTile && t <= kMaxTile && t <= n, EV_E_ARG, "fused pass tile width out of range");
     EV_REQUIRE(n - t < 32, EV_E_LIMIT, "fused pass has more than 2^32 - 1 tiles");
-    if (!c64) launch_pass_t<double>(psi, n, t, pass, stages, ops, coef, st);
-    else launch_pass_t<float>(psi, n, t, pass, stages, ops, coef, st);
+    EV_REQUIRE(c64 || !mfma_u2, EV_E_ARG, "matrix-core dense gates are a complex64 option");
+    if (!c64) launch_pass_t<double, false>(psi, n, t, pass, stages, ops, coef, st);
+    else if (mfma_u2) launch_pass_t<float, true>(psi, n, t, pass, stages, ops, coef, st);
+    else launch_pass_t<float, false>(psi, n, t, pass, stages, ops, coef, st);
 }
 
 void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st)

@@ -10,9 +10,10 @@ namespace ev::fused {
 // One launch = one pass over a 2^n state with tile width t: 2^(n-t)
 // workgroups of 2^(t-4) threads, one tile of 2^t amplitudes each.
 // Descriptor pointers are device memory; coef holds double2 (c128) or float2
-// (c64) entries.
+// (c64) entries.  mfma_u2 (c64 only) runs dense 4x4 gates on the matrix
+// cores.
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
-                 const void *coef, hipStream_t st);
+                 const void *coef, bool mfma_u2, hipStream_t st);
 
 // One entry of a batched round: a run of consecutive tiles of one state's next
 // pass, from tile tile_off on.  tile_begin is the prefix sum of the tile

@@ -1305,6 +1305,8 @@ FusedConfig fused_config(int n, bool c64, const ev_run_options &opt)
     f.t = std::min(f.t, n);
     f.low = std::min(f.low, f.t - 1);
     f.coalesce = c64 ? 4 : 3;  // 128-byte runs
+    EV_REQUIRE(c64 || !opt.mfma_u2, EV_E_ARG, "mfma_u2 applies to complex64 runs");
+    f.mfma_u2 = opt.mfma_u2 != 0;
     return f;
 }
 
@@ -1319,7 +1321,7 @@ namespace {
 // drained (every earlier copy has landed) and the ring starts over.
 class PassStreamer {
   public:
-    PassStreamer(ev_sv *s, RunStats &st, Profiler &prof) : s_(s), st_(st), prof_(prof) {}
+    PassStreamer(ev_sv *s, bool mfma_u2, RunStats &st, Profiler &prof) : s_(s), mfma_(mfma_u2), st_(st), prof_(prof) {}
 
     void operator()(const Plan &P, size_t p)
     {
@@ -1369,7 +1371,7 @@ class PassStreamer {
         prof_.begin();
         fused::launch_pass(s_->psi, c64, s_->n, ps.t, reinterpret_cast<const Pass *>(dev),
                            reinterpret_cast<const Stage *>(dev + o_stage), reinterpret_cast<const DevOp *>(dev + o_ops),
-                           dev + o_coef, s_->stream);
+                           dev + o_coef, mfma_, s_->stream);
         prof_.end("fused_pass", full_pass_bytes(s_->n, ab));
         st_.pass_flops += fpa * std::ldexp(1.0, s_->n);
         ++st_.passes;
@@ -1403,6 +1405,7 @@ class PassStreamer {
     }
 
     ev_sv *s_;
+    bool mfma_;
     RunStats &st_;
     Profiler &prof_;
     size_t coef_done_ = 0;
@@ -1450,7 +1453,7 @@ void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st
     const std::vector<std::pair<int, int>> tail = peel_swap_tail(low, cfg, s->n);
     const float budget = (cfg.t >= s->n) ? 1e30f : cfg.budget;  // one tile holds the state: one pass
     if (!low.ops.empty()) {
-        PassStreamer launch(s, st, prof);
+        PassStreamer launch(s, cfg.mfma_u2, st, prof);
         double in_sink = 0;  // host time spent uploading and launching, not planning
         plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce, [&](const Plan &P, size_t p) {
             const double a = now_ms();

@@ -23,6 +23,7 @@ struct FusedConfig {
     int t, low;
     float budget;
     int coalesce;
+    bool mfma_u2;  // complex64: dense 4x4 gates on the matrix cores
 };
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt);
 

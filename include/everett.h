@@ -199,6 +199,10 @@ typedef struct {
                                launch it once */
     uint32_t renormalize;   /* batch: rescale every state to unit f64 norm at the end (c64 drift) */
     uint32_t low_qubits;    /* lowest qubits always in a tile (contiguous rows); 0 = tuned or default */
+    uint32_t mfma_u2;       /* complex64 single runs: 1 = dense 4x4 gates on the matrix cores
+                               (v_mfma_f32_4x4x1_16b_f32) instead of VALU FMA chains; 0 = VALU.
+                               Slower on MI355X at 30 qubits: the kernel then holds 116 VGPRs
+                               (4 waves per SIMD instead of 6) */
 } ev_run_options;
 
 /* src/backend/statevector.rs:68-88: validate, |0..0>, Rng::seed_from_u64(seed),

@@ -71,6 +71,25 @@ def test_fused_c64_fidelity(n, tile):
     assert np.max(np.abs(got - want)) < 1e-5 and 1 - fid < 1e-6
 
 
+@pytest.mark.parametrize("n,tile,seed", [(12, 0, 0), (18, 0, 0), (18, 13, 0), (15, 0, 3)])
+def test_fused_c64_matrix_core_gates(n, tile, seed):
+    # dense 4x4 gates on v_mfma_f32_4x4x1_16b_f32 (the mfma_u2 run option): same bounds as the VALU path,
+    # on brick-walls (dense 4x4 blocks) and on mixed circuits (every op kind beside them)
+    c = ev.workloads.config1(n, 20) if seed == 0 else mixed_circuit(n, 300, seed)
+    want, _ = oracle.sv_run(c.ops(), n)
+    s = ev.StateVectorBackend.run(c, precision="c64", tile_qubits=tile, mfma_u2=True).state()
+    got = s.amplitudes()
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    assert np.max(np.abs(got - want)) < 1e-5 and 1 - fid < 1e-6
+    valu = ev.StateVectorBackend.run(c, precision="c64", tile_qubits=tile).state().amplitudes()
+    assert np.max(np.abs(got - valu)) < 1e-5
+
+
+def test_matrix_core_gates_are_a_complex64_option():
+    with pytest.raises(ev.EverettError):
+        ev.StateVectorBackend.run(ev.workloads.config1(12, 4), precision="c128", mfma_u2=True)
+
+
 @pytest.mark.parametrize("n", [24, 28])
 def test_fused_matches_plain_at_scale(n):
     # the plain path equals the oracle exactly (test_gpu_backends), so it is the reference here

@@ -38,6 +38,36 @@ def test_op_layout_matches_header():
     assert _ffi.OP_DTYPE.fields["m"][1] == 152
 
 
+STRUCT_MIRRORS = {  # C struct in include/everett.h -> its ctypes mirror in everett_amd/_ffi.py
+    "ev_error": "EvError", "ev_rng": "EvRng", "ev_run_options": "EvRunOptions", "ev_channel": "EvChannel",
+    "ev_noise_model": "EvNoiseModel", "ev_run_stats": "EvRunStats", "ev_kernel_stat": "EvKernelStat",
+    "ev_batch_stats": "EvBatchStats", "ev_server_config": "EvServerConfig", "ev_request_info": "EvRequestInfo",
+    "ev_server_stats": "EvServerStats", "ev_tune_entry": "EvTuneEntry", "ev_plan_info": "EvPlanInfo",
+}
+
+
+def test_struct_mirrors_match_header(tmp_path):
+    # sizeof and every field offset as the C compiler lays the header out, against ctypes
+    import subprocess
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "everett.h"', "int main(void) {"]
+    for cname, pyname in STRUCT_MIRRORS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in getattr(_ffi, pyname)._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for cname, pyname in STRUCT_MIRRORS.items():
+        cls = getattr(_ffi, pyname)
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
+
+
 # ------------------------------------------------------------- validation
 
 def test_validate_order_and_errors():  # circuit.rs:252-306 and its tests
