@@ -375,12 +375,20 @@ def bench_c4(args) -> dict:
     span_s = (max(i["done_ms"] for i in infos) - min(i["submit_ms"] for i in infos)) * 1e-3
     served = R / span_s
 
-    # the largest request on an idle server: its service time bounds the tail of the stream from below
-    big = max(range(R), key=lambda k: (reqs[k][0].num_qubits(), len(reqs[k][0])))
+    # the four largest requests (gates x 2^n) each on an idle server: every latency is at least its request's
+    # service time, so the k-th largest latency is at least the k-th largest service time
+    work = lambda k: len(reqs[k][0]) * 2.0 ** reqs[k][0].num_qubits()
+    biggest = sorted(range(R), key=work, reverse=True)[:4]
+    alone = []
     with ev.Server(precision="c64") as srv:
-        t2 = time.perf_counter()
-        srv.wait(srv.submit(reqs[big][0], shots=1024, seed=reqs[big][1]))
-        big_alone_ms = (time.perf_counter() - t2) * 1e3
+        for k in biggest:
+            t2 = time.perf_counter()
+            srv.wait(srv.submit(reqs[k][0], shots=1024, seed=reqs[k][1]))
+            alone.append({"request": k, "qubits": reqs[k][0].num_qubits(), "gates": len(reqs[k][0]),
+                          "arrival_ms": arrivals[k] * 1e3, "alone_ms": (time.perf_counter() - t2) * 1e3,
+                          "latency_ms": float(lat[k])})
+    big = biggest[0]
+    big_alone_ms = alone[0]["alone_ms"]
 
     # single-request plain-GPU baseline: the same requests one at a time, one launch per gate
     t1 = time.perf_counter()
@@ -424,6 +432,10 @@ def bench_c4(args) -> dict:
         "latency_by_qubits": by_n,
         "largest_request_alone_ms": big_alone_ms,
         "largest_request": f"request {big}: {reqs[big][0].num_qubits()} qubits, {len(reqs[big][0])} gates, on an idle server",
+        "largest_requests": alone,
+        "p99_floor_note": "every latency is at least its own service time (largest_requests.alone_ms), and p99 over "
+                          f"{R} requests interpolates between the 4th and 3rd largest latencies; requests that "
+                          "arrive while a larger one is still running add their service times to each other's",
         "fidelity_vs_f64": fid,
     }
 
