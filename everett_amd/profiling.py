@@ -155,8 +155,32 @@ def positions(args) -> int:
     return 0
 
 
-SQ_COUNTERS = ("SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAIT_INST_ANY",
-               "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT")
+# Columns of the SQ summary: (header, numerator counter, denominator) where the
+# denominator is "waves" (per wave of the dispatch), a counter name, or
+# "simd-cycles" (GRBM_GUI_ACTIVE x SIMDs: a fraction of the device's SIMD time).
+# A column appears when its counters were collected.
+SQ_COLUMNS = (
+    ("VALU/wave", "SQ_INSTS_VALU", "waves"),
+    ("MFMA/wave", "SQ_INSTS_MFMA", "waves"),
+    ("LDS/wave", "SQ_INSTS_LDS", "waves"),
+    ("bankconf/wave", "SQ_LDS_BANK_CONFLICT", "waves"),
+    ("VALU-active", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+    ("wait-any", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+    ("wait-lds", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("MFMA-busy", "SQ_VALU_MFMA_BUSY_CYCLES", "simd-cycles"),
+)
+SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
+
+
+def _ratio(c: dict, num: str, den: str, waves: float):
+    if num not in c:
+        return None
+    if den == "waves":
+        return c[num] / waves if waves else None
+    if den == "simd-cycles":
+        g = c.get("GRBM_GUI_ACTIVE")
+        return c[num] / (g * SIMDS) if g else None
+    return c[num] / c[den] if c.get(den) else None
 
 
 def sq(args) -> int:
@@ -175,30 +199,34 @@ def sq(args) -> int:
             per.setdefault(d, {})[c] = float(v)
     if not per:
         raise SystemExit(f"no dispatches of a kernel matching {args.kernel!r}")
+    names = sorted({k for c in per.values() for k in c})
+    cols = [col for col in SQ_COLUMNS if col[1] in names and (col[2] in ("waves", "simd-cycles") or col[2] in names)]
     lines = [f"# {args.title}", "", f"command: {args.command}",
              f"source: rocprofv3 --pmc ({Path(args.db).name}), {len(per)} dispatches of {args.kernel}", "",
-             "per dispatch (waves = grid threads / 64; SQ_WAVE_CYCLES and SQ_ACTIVE_INST_VALU count in the SQ's",
-             "own units, so their ratio is the fraction of a wave's life spent issuing VALU):", "",
-             f"{'dispatch':>8} {'us':>9} {'VALU/wave':>10} {'LDS/wave':>9} {'bankconf/wave':>14} "
-             f"{'VALU-active':>12} {'wait-any':>9}"]
-    tot = {c: 0.0 for c in SQ_COUNTERS}
+             "per dispatch (waves = grid threads / 64; SQ_WAVE_CYCLES and the SQ_ACTIVE_/SQ_WAIT_ counters count in",
+             "the SQ's own units, so their ratios are fractions of a wave's life; MFMA-busy is",
+             f"SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x {SIMDS} SIMDs)):", "",
+             f"{'dispatch':>8} {'us':>9} " + " ".join(f"{h:>13}" for h, _, _ in cols)]
+    tot = {k: 0.0 for k in names}
     waves_tot = 0.0
     for d in sorted(per):
         c = per[d]
         gx, _ = grid.get(d, (0, 64))
         waves = gx / 64.0
         waves_tot += waves
-        for k in SQ_COUNTERS:
+        for k in names:
             tot[k] += c.get(k, 0.0)
-        lines.append(f"{d:>8} {dur.get(d, 0.0):>9.1f} {c['SQ_INSTS_VALU'] / waves:>10.0f} "
-                     f"{c['SQ_INSTS_LDS'] / waves:>9.0f} {c['SQ_LDS_BANK_CONFLICT'] / waves:>14.0f} "
-                     f"{c['SQ_ACTIVE_INST_VALU'] / c['SQ_WAVE_CYCLES']:>12.3f} "
-                     f"{c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:>9.3f}")
-    lines += ["", "all dispatches: " + ", ".join(f"{k} {tot[k]:.4g}" for k in SQ_COUNTERS),
-              f"VALU instructions per wave {tot['SQ_INSTS_VALU'] / waves_tot:.0f}, "
-              f"LDS instructions per wave {tot['SQ_INSTS_LDS'] / waves_tot:.0f}, "
-              f"VALU-active fraction of wave cycles {tot['SQ_ACTIVE_INST_VALU'] / tot['SQ_WAVE_CYCLES']:.3f}, "
-              f"waiting on any instruction {tot['SQ_WAIT_INST_ANY'] / tot['SQ_WAVE_CYCLES']:.3f}"]
+        cells = []
+        for _, num, den in cols:
+            r = _ratio(c, num, den, waves)
+            cells.append(f"{'-':>13}" if r is None else (f"{r:>13.0f}" if den == "waves" else f"{r:>13.3f}"))
+        lines.append(f"{d:>8} {dur.get(d, 0.0):>9.1f} " + " ".join(cells))
+    summary = []
+    for h, num, den in cols:
+        r = _ratio(tot, num, den, waves_tot)
+        if r is not None:
+            summary.append(f"{h} {r:.0f}" if den == "waves" else f"{h} {r:.3f}")
+    lines += ["", "all dispatches: " + ", ".join(f"{k} {tot[k]:.4g}" for k in names), "totals: " + ", ".join(summary)]
     text = "\n".join(lines) + "\n"
     Path(args.out).write_text(text)
     print(text, end="")

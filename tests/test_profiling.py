@@ -62,3 +62,29 @@ def test_traffic_scaling(tmp_path):
     assert e["fetch_bytes_per_launch"] == pytest.approx(2 << 20)  # 1024 KiB x scale 2
     assert e["write_bytes_per_launch"] == pytest.approx(2 << 20)  # 2048 KiB x scale 1
     assert e["traffic_over_algorithmic"] == pytest.approx(4 / 3)
+
+
+def test_sq_summary_per_wave_and_fractions(tmp_path):
+    db = tmp_path / "q.db"
+    con = sqlite3.connect(str(db))
+    con.execute("CREATE TABLE kernels (dispatch_id INTEGER, name TEXT, start INTEGER, end INTEGER, grid_x INTEGER, "
+                "workgroup_x INTEGER)")
+    con.execute("CREATE TABLE counters_collection (dispatch_id INTEGER, kernel_name TEXT, counter_name TEXT, "
+                "value REAL)")
+    con.executemany("INSERT INTO kernels VALUES (?, ?, ?, ?, ?, ?)",
+                    [(1, "k_fused_pass<float, true>", 0, 2_000_000, 6400, 256), (2, "k_copy", 0, 1000, 64, 64)])
+    vals = {"SQ_INSTS_VALU": 5000.0, "SQ_INSTS_MFMA": 1000.0, "SQ_WAVE_CYCLES": 400.0, "SQ_ACTIVE_INST_VALU": 100.0,
+            "SQ_VALU_MFMA_BUSY_CYCLES": 512.0, "GRBM_GUI_ACTIVE": 2.0}
+    con.executemany("INSERT INTO counters_collection VALUES (?, ?, ?, ?)",
+                    [(1, "k_fused_pass<float, true>", k, v) for k, v in vals.items()] +
+                    [(2, "k_copy", "SQ_INSTS_VALU", 7.0)])
+    con.commit()
+    con.close()
+    out = tmp_path / "q.txt"
+    prof.main(["sq", "--db", str(db), "--command", "rocprofv3 --pmc ...", "--kernel", "k_fused_pass",
+               "--out", str(out)])
+    text = out.read_text()
+    row = [l for l in text.splitlines() if l.strip().startswith("1 ")][0].split()
+    # 100 waves: VALU 50 per wave, MFMA 10 per wave, VALU-active 0.25, MFMA busy 512 / (2 x 1024) = 0.25
+    assert row[2:] == ["50", "10", "0.250", "0.250"]
+    assert "MFMA-busy 0.250" in text and "k_copy" not in text
