@@ -1328,6 +1328,7 @@ class PassStreamer {
         const bool c64 = s_->prec == kern::C64;
         const size_t ab = kern::amp_bytes(s_->prec);
         const Pass &ps = P.passes[p];
+        EV_REQUIRE(ps.nstages > 0, EV_E_STATE, "fused pass without stages");
         const Stage *stg = P.stages.data() + ps.stage_begin;
         const int op0 = stg[0].op_begin, op1 = stg[ps.nstages - 1].op_end;
         const size_t coef0 = coef_done_, coef1 = P.coef.size();
