@@ -156,9 +156,12 @@ def positions(args) -> int:
 
 
 # Columns of the SQ summary: (header, numerator counter, denominator) where the
-# denominator is "waves" (per wave of the dispatch), a counter name, or
-# "simd-cycles" (GRBM_GUI_ACTIVE x SIMDs: a fraction of the device's SIMD time).
-# A column appears when its counters were collected.
+# denominator is "waves" (per wave of the dispatch), a counter name,
+# "simd-cycles" (a fraction of the device's SIMD time), "simd-waves" (mean
+# resident waves per SIMD: SQ_WAVE_CYCLES counts 4-cycle units) or "clock"
+# (GHz over the dispatch).  The database sums GRBM_GUI_ACTIVE over the 8 XCDs,
+# so one XCD's active cycles are a eighth of it.  A column appears when its
+# counters were collected.
 SQ_COLUMNS = (
     ("VALU/wave", "SQ_INSTS_VALU", "waves"),
     ("MFMA/wave", "SQ_INSTS_MFMA", "waves"),
@@ -168,18 +171,25 @@ SQ_COLUMNS = (
     ("wait-any", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
     ("wait-lds", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
     ("MFMA-busy", "SQ_VALU_MFMA_BUSY_CYCLES", "simd-cycles"),
+    ("waves/SIMD", "SQ_WAVE_CYCLES", "simd-waves"),
+    ("GHz", "GRBM_GUI_ACTIVE", "clock"),
 )
 SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
+XCDS = 8
 
 
-def _ratio(c: dict, num: str, den: str, waves: float):
+def _ratio(c: dict, num: str, den: str, waves: float, us: float = 0.0):
     if num not in c:
         return None
+    g = c.get("GRBM_GUI_ACTIVE", 0.0) / XCDS  # one XCD's active cycles
     if den == "waves":
         return c[num] / waves if waves else None
     if den == "simd-cycles":
-        g = c.get("GRBM_GUI_ACTIVE")
         return c[num] / (g * SIMDS) if g else None
+    if den == "simd-waves":
+        return 4.0 * c[num] / (g * SIMDS) if g else None
+    if den == "clock":
+        return g / (us * 1e3) if us else None
     return c[num] / c[den] if c.get(den) else None
 
 
@@ -200,12 +210,16 @@ def sq(args) -> int:
     if not per:
         raise SystemExit(f"no dispatches of a kernel matching {args.kernel!r}")
     names = sorted({k for c in per.values() for k in c})
-    cols = [col for col in SQ_COLUMNS if col[1] in names and (col[2] in ("waves", "simd-cycles") or col[2] in names)]
+    derived = ("waves", "simd-cycles", "simd-waves", "clock")
+    cols = [col for col in SQ_COLUMNS if col[1] in names and (col[2] in derived or col[2] in names)
+            and (col[2] not in derived[1:] or "GRBM_GUI_ACTIVE" in names)]
     lines = [f"# {args.title}", "", f"command: {args.command}",
              f"source: rocprofv3 --pmc ({Path(args.db).name}), {len(per)} dispatches of {args.kernel}", "",
              "per dispatch (waves = grid threads / 64; SQ_WAVE_CYCLES and the SQ_ACTIVE_/SQ_WAIT_ counters count in",
-             "the SQ's own units, so their ratios are fractions of a wave's life; MFMA-busy is",
-             f"SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x {SIMDS} SIMDs)):", "",
+             "the SQ's own 4-cycle units, so their ratios are fractions of a wave's life; with G = GRBM_GUI_ACTIVE / "
+             f"{XCDS} (one XCD's",
+             f"active cycles), MFMA-busy = SQ_VALU_MFMA_BUSY_CYCLES / (G x {SIMDS} SIMDs), waves/SIMD = "
+             f"4 SQ_WAVE_CYCLES / (G x {SIMDS}), GHz = G / duration):", "",
              f"{'dispatch':>8} {'us':>9} " + " ".join(f"{h:>13}" for h, _, _ in cols)]
     tot = {k: 0.0 for k in names}
     waves_tot = 0.0
@@ -218,12 +232,13 @@ def sq(args) -> int:
             tot[k] += c.get(k, 0.0)
         cells = []
         for _, num, den in cols:
-            r = _ratio(c, num, den, waves)
+            r = _ratio(c, num, den, waves, dur.get(d, 0.0))
             cells.append(f"{'-':>13}" if r is None else (f"{r:>13.0f}" if den == "waves" else f"{r:>13.3f}"))
         lines.append(f"{d:>8} {dur.get(d, 0.0):>9.1f} " + " ".join(cells))
     summary = []
+    us_tot = sum(dur.get(d, 0.0) for d in per)
     for h, num, den in cols:
-        r = _ratio(tot, num, den, waves_tot)
+        r = _ratio(tot, num, den, waves_tot, us_tot)
         if r is not None:
             summary.append(f"{h} {r:.0f}" if den == "waves" else f"{h} {r:.3f}")
     lines += ["", "all dispatches: " + ", ".join(f"{k} {tot[k]:.4g}" for k in names), "totals: " + ", ".join(summary)]

@@ -73,8 +73,9 @@ def test_sq_summary_per_wave_and_fractions(tmp_path):
                 "value REAL)")
     con.executemany("INSERT INTO kernels VALUES (?, ?, ?, ?, ?, ?)",
                     [(1, "k_fused_pass<float, true>", 0, 2_000_000, 6400, 256), (2, "k_copy", 0, 1000, 64, 64)])
+    # GRBM_GUI_ACTIVE summed over 8 XCDs: 16 -> 2 active cycles per XCD over the 2 ms dispatch
     vals = {"SQ_INSTS_VALU": 5000.0, "SQ_INSTS_MFMA": 1000.0, "SQ_WAVE_CYCLES": 400.0, "SQ_ACTIVE_INST_VALU": 100.0,
-            "SQ_VALU_MFMA_BUSY_CYCLES": 512.0, "GRBM_GUI_ACTIVE": 2.0}
+            "SQ_VALU_MFMA_BUSY_CYCLES": 512.0, "GRBM_GUI_ACTIVE": 16.0}
     con.executemany("INSERT INTO counters_collection VALUES (?, ?, ?, ?)",
                     [(1, "k_fused_pass<float, true>", k, v) for k, v in vals.items()] +
                     [(2, "k_copy", "SQ_INSTS_VALU", 7.0)])
@@ -85,6 +86,7 @@ def test_sq_summary_per_wave_and_fractions(tmp_path):
                "--out", str(out)])
     text = out.read_text()
     row = [l for l in text.splitlines() if l.strip().startswith("1 ")][0].split()
-    # 100 waves: VALU 50 per wave, MFMA 10 per wave, VALU-active 0.25, MFMA busy 512 / (2 x 1024) = 0.25
-    assert row[2:] == ["50", "10", "0.250", "0.250"]
+    # 100 waves: VALU 50 per wave, MFMA 10 per wave, VALU-active 0.25, MFMA busy 512 / (2 x 1024) = 0.25,
+    # resident waves 4 x 400 / (2 x 1024), clock 2 cycles / 2 ms
+    assert row[2:] == ["50", "10", "0.250", "0.250", "0.781", "0.000"]
     assert "MFMA-busy 0.250" in text and "k_copy" not in text
