@@ -85,6 +85,22 @@ def test_fused_c64_matrix_core_gates(n, tile, seed):
     assert np.max(np.abs(got - valu)) < 1e-5
 
 
+def test_pass_descriptor_ring_wraps_and_grows():
+    # passes stream to the device through a per-state ring (1 MiB below 20 qubits): ~1,000 passes of
+    # ~2.5 KB each wrap it twice in one run; one pass of ~1.4 MB outgrows it after a smaller run used it
+    s = ev.State.zero(14)
+    c = mixed_circuit(14, 3000, 11)
+    ev.run_into(s, c, 0, tile_qubits=10, max_gates_per_pass=2)
+    want, _ = oracle.sv_run(c.ops(), 14)
+    assert np.max(np.abs(s.amplitudes() - want)) < TOL128
+    s = ev.State.zero(12)
+    small, big = mixed_circuit(12, 200, 12), mixed_circuit(12, 12000, 13)
+    for circ in (small, big, small):
+        ev.run_into(s, circ, 0)
+        want, _ = oracle.sv_run(circ.ops(), 12)
+        assert np.max(np.abs(s.amplitudes() - want)) < 1e-9
+
+
 def test_matrix_core_gates_are_a_complex64_option():
     with pytest.raises(ev.EverettError):
         ev.StateVectorBackend.run(ev.workloads.config1(12, 4), precision="c128", mfma_u2=True)
