@@ -94,6 +94,7 @@ def lib():
             "ref_dm_renormalize": (None, [P, SZ]),
             "ref_channel_kraus": (SZ, [I, D, P, SZ, P]),
             "ref_dm_run": (I, [P, SZ, SZ, C.POINTER(RefNoise), U64, P, P, SZ]),
+            "ref_dm_run_superop": (I, [P, SZ, SZ, C.POINTER(RefNoise), U64, P, P, SZ]),
             "ref_gen_qft": (SZ, [SZ, P]),
             "ref_gen_brickwall": (SZ, [SZ, SZ, R, P]),
             "ref_gen_dm_random": (SZ, [SZ, SZ, R, P]),
@@ -300,8 +301,9 @@ def dm_purity(rho: np.ndarray, n: int) -> float:
     return float(lib().ref_dm_purity(_p(rho), n))
 
 
-def dm_run(ops: np.ndarray, n: int, noise=None, seed: int = 0, num_classical: int = 0):
-    """noise: dict(kind_1q, p_1q, custom_1q, kind_2q, p_2q, custom_2q, readout_error)."""
+def dm_run(ops: np.ndarray, n: int, noise=None, seed: int = 0, num_classical: int = 0, superop: bool = False):
+    """noise: dict(kind_1q, p_1q, custom_1q, kind_2q, p_2q, custom_2q, readout_error).  superop=True runs
+    ref_dm_run_superop (every gate and channel as a pass over the 2n-qubit vector; parallel)."""
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     noise = noise or {}
     keep = []
@@ -322,8 +324,8 @@ def dm_run(ops: np.ndarray, n: int, noise=None, seed: int = 0, num_classical: in
     dim = 1 << n
     rho = np.zeros(dim * dim, dtype=np.complex128)
     bits = np.zeros(max(num_classical, 1), dtype=np.uint8)
-    rc = lib().ref_dm_run(_p(ops), len(ops), n, C.byref(nz), seed & 0xFFFFFFFFFFFFFFFF, _p(rho), _p(bits),
-                          num_classical)
+    fn = lib().ref_dm_run_superop if superop else lib().ref_dm_run
+    rc = fn(_p(ops), len(ops), n, C.byref(nz), seed & 0xFFFFFFFFFFFFFFFF, _p(rho), _p(bits), num_classical)
     if rc != 0:
         raise ValueError("malformed op array or noise model")
     return rho, [bool(b) for b in bits[:num_classical]]

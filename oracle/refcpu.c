@@ -683,6 +683,118 @@ int ref_dm_run(const ref_op *ops, size_t nops, size_t n, const ref_noise *noise,
     return rc;
 }
 
+/* Superoperator form of the same backend (not in the reference: a faster
+ * restatement for large n, checked against ref_dm_run at small n).  rho's
+ * row-major storage is the 2n-qubit vector whose bit n+t is row qubit t and
+ * bit t column qubit t, so U rho U^dag is U on the row bits then conj(U) on
+ * the column bits (density.rs:164-191 entry by entry), and a channel
+ * sum_k K rho K^dag (density.rs:198-216) is the 4x4 S = sum_k K (x) conj(K) on
+ * bits (n+t, t): S[(r,c),(r',c')] = sum_k K[r][r'] conj(K[c][c']).  Sums run
+ * in another order than the literal loops, so results agree to rounding. */
+static void conj_n(const rc128 *m, size_t cnt, rc128 *out)
+{
+    for (size_t i = 0; i < cnt; ++i) out[i] = cconj(m[i]);
+}
+
+/* S = sum_k A_k (x) conj(B_k) on (row bit, column bit), the 4x4 acting on
+ * vec(rho) as rho -> sum_k A_k rho B_k^dag (A = B = K for a channel, A = B =
+ * U with one term for a unitary). */
+static void superop(const rc128 *a, const rc128 *b, size_t nk, rc128 s[16])
+{
+    for (int i = 0; i < 16; ++i) s[i] = C(0, 0);
+    for (size_t k = 0; k < nk; ++k) {
+        const rc128 *A = a + 4 * k, *B = b + 4 * k;
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 2; ++c)
+                for (int r2 = 0; r2 < 2; ++r2)
+                    for (int c2 = 0; c2 < 2; ++c2)
+                        s[(2 * r + c) * 4 + (2 * r2 + c2)] =
+                            cadd(s[(2 * r + c) * 4 + (2 * r2 + c2)], cmul(A[2 * r + r2], cconj(B[2 * c + c2])));
+    }
+}
+
+static void mat4_mul(const rc128 x[16], const rc128 y[16], rc128 out[16])
+{
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            rc128 acc = C(0, 0);
+            for (int k = 0; k < 4; ++k) acc = cadd(acc, cmul(x[4 * r + k], y[4 * k + c]));
+            out[4 * r + c] = acc;
+        }
+}
+
+static void dmv_channel(dm_ctx *d, const rc128 *kraus, size_t nk, size_t t)
+{
+    rc128 s[16];
+    superop(kraus, kraus, nk, s);
+    ref_apply_2q(d->rho, (size_t)1 << (2 * d->n), d->n + t, t, s);
+}
+
+/* A one-qubit gate and the channel after it are one pass: S_K (U (x) conj(U)). */
+static void dmv_1q(void *c, const rc128 m[4], size_t t)
+{
+    dm_ctx *d = c;
+    rc128 su[16], sk[16], s[16];
+    superop(m, m, 1, su);
+    if (d->nk1) {
+        superop(d->k1, d->k1, d->nk1, sk);
+        mat4_mul(sk, su, s);
+    } else {
+        memcpy(s, su, sizeof s);
+    }
+    ref_apply_2q(d->rho, (size_t)1 << (2 * d->n), d->n + t, t, s);
+}
+
+static void dmv_2q(void *c, const rc128 m[16], size_t a, size_t b)
+{
+    dm_ctx *d = c;
+    const size_t len = (size_t)1 << (2 * d->n);
+    rc128 mc[16];
+    conj_n(m, 16, mc);
+    ref_apply_2q(d->rho, len, d->n + a, d->n + b, m);
+    ref_apply_2q(d->rho, len, a, b, mc);
+    if (d->nk2) {
+        dmv_channel(d, d->k2, d->nk2, a);
+        dmv_channel(d, d->k2, d->nk2, b);
+    }
+}
+
+static void dmv_ctl(void *c, const uint64_t *cs, size_t nc, const rc128 m[4], size_t t)
+{
+    dm_ctx *d = c;
+    const size_t len = (size_t)1 << (2 * d->n);
+    uint64_t rows[64];
+    rc128 mc[4];
+    if (nc > 64) return;
+    for (size_t i = 0; i < nc; ++i) rows[i] = cs[i] + d->n;
+    conj_n(m, 4, mc);
+    ref_apply_controlled_1q(d->rho, len, rows, nc, d->n + t, m);
+    ref_apply_controlled_1q(d->rho, len, cs, nc, t, mc);
+    if (d->nk1) {
+        dmv_channel(d, d->k1, d->nk1, t);
+        for (size_t i = 0; i < nc; ++i) dmv_channel(d, d->k1, d->nk1, cs[i]);
+    }
+}
+static const backend_vt DMV_VT = {dmv_1q, dmv_2q, dmv_ctl, dm_meas};
+
+int ref_dm_run_superop(const ref_op *ops, size_t nops, size_t n, const ref_noise *noise, uint64_t seed, rc128 *rho,
+                       uint8_t *classical, size_t ncl)
+{
+    size_t dim = (size_t)1 << n;
+    memset(rho, 0, dim * dim * sizeof(rc128));
+    rho[0] = C(1, 0);
+    ref_rng rng;
+    ref_rng_seed(&rng, seed);
+    dm_ctx *d = calloc(1, sizeof(dm_ctx));
+    d->rho = rho; d->n = n; d->rng = &rng; d->noise = noise;
+    if (noise->ncustom_1q > 64 || noise->ncustom_2q > 64) { free(d); return -1; }
+    d->nk1 = ref_channel_kraus(noise->kind_1q, noise->p_1q, noise->custom_1q, noise->ncustom_1q, d->k1);
+    d->nk2 = ref_channel_kraus(noise->kind_2q, noise->p_2q, noise->custom_2q, noise->ncustom_2q, d->k2);
+    int rc = drive(&DMV_VT, d, ops, nops, classical, ncl);
+    free(d);
+    return rc;
+}
+
 /* -------------------------------------------- algorithms/qft.rs, generators */
 
 static ref_op *put1(ref_op *out, size_t *cnt, const rc128 m[4], size_t t)

@@ -122,6 +122,11 @@ typedef struct {
 } ref_noise;
 int ref_dm_run(const ref_op *ops, size_t nops, size_t n, const ref_noise *noise, uint64_t seed,
                rc128 *rho, uint8_t *classical, size_t ncl);             /* density_matrix.rs:97-113 */
+/* The same run with every gate and channel as a pass over rho viewed as a
+ * 2n-qubit vector (superoperators); equal to ref_dm_run to rounding, and
+ * parallel (ref_set_threads), for sizes the literal loops cannot reach. */
+int ref_dm_run_superop(const ref_op *ops, size_t nops, size_t n, const ref_noise *noise, uint64_t seed,
+                       rc128 *rho, uint8_t *classical, size_t ncl);
 
 /* ---- algorithms/qft.rs and the workload generators ---- */
 /* Each returns the number of ops written (call with out=NULL to count). */

@@ -11,8 +11,14 @@ fused run is checked against the plain (one superoperator pass per gate and
 channel) run to 1e-10, and the result against what a density matrix must
 satisfy: unit trace, Hermiticity, a non-negative diagonal and purity in (0, 1].
 Both paths are pinned to the oracle at small sizes in test_gpu_backends.py and
-test_gpu_fused.py.
+test_gpu_fused.py.  The oracle's superoperator restatement (the same gates and
+channels as passes over the 2n-qubit vector, on all host cores; pinned to the
+literal loops at n <= 8 in test_oracle.py) reaches 13 qubits in well under a
+minute: the fused noisy state is held to it at 1e-10, and the configured run's
+classical record to the reference's measurement sequence on its diagonal.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -95,3 +101,39 @@ def test_config2_at_13_qubits_fused_equals_plain_and_is_a_density_matrix():
     assert abs(a[x, x] - 1.0) <= 1e-10
     a[x, x] = 0
     assert np.max(np.abs(a)) <= 1e-12
+
+
+def _measure_all(diag, n, seed, readout):
+    """density_matrix.rs:155-169 on the diagonal of rho, qubits 0..n-1 in order: p1 a sequential sum,
+    raw = u < p1, collapse, 1/trace, then the readout draw (only when readout > 0)."""
+    d = np.array(diag, dtype=np.float64)
+    r = oracle.Rng(seed)
+    idx = np.arange(len(d))
+    rec = []
+    for q in range(n):
+        hit = (idx >> q) & 1 == 1
+        p1 = float(np.cumsum(d[hit])[-1])
+        raw = r.next_f64() < p1
+        d[hit != raw] = 0.0
+        d = d * (1.0 / float(np.cumsum(d)[-1]))
+        rec.append((not raw) if readout > 0 and r.next_f64() < readout else raw)
+    return rec
+
+
+def test_config2_at_13_qubits_against_the_superoperator_oracle():
+    n, seed = 13, 13
+    c, noise = ev.workloads.config2(n, 30)
+    body = ev.workloads.dm_random(n, 30, ev.Rng.seed_from_u64(n))
+    ks = np.array([k.m for k in noise.after_1q.ops])
+    ref = {"custom_1q": ks, "custom_2q": ks, "readout_error": 0.01}
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        want, _ = oracle.dm_run(body.ops(), n, ref, seed, superop=True)
+    finally:
+        oracle.set_threads(1)
+    got = ev.DensityMatrixBackend.run_with_noise_seeded(body, noise, seed).density_matrix().matrix().reshape(-1)
+    assert np.max(np.abs(got - want)) <= 1e-10
+    diag = want.reshape(1 << n, 1 << n).diagonal().real.copy()
+    del want, got
+    ex = ev.DensityMatrixBackend.run_with_noise_seeded(c, noise, seed)
+    assert ex.classical() == _measure_all(diag, n, seed, 0.01)

@@ -553,6 +553,31 @@ def test_dm_controlled_matches_brute_force():  # density_matrix.rs:140-153, 212-
     assert np.max(np.abs(rho.reshape(8, 8) - np.outer(psi, psi.conj()))) < 1e-12
 
 
+@pytest.mark.parametrize("n,seed", [(2, 1), (4, 2), (6, 3), (8, 4)])
+def test_dm_superop_run_matches_the_literal_loops(n, seed):
+    # ref_dm_run_superop (gates and channels as passes over the 2n-qubit vector) against ref_dm_run
+    # (the reference's loops): dm_random circuits plus multi-controlled gates and mid-circuit
+    # measurements, with no noise, named channels, and a custom Kraus set with readout error
+    rng = np.random.default_rng(seed)
+    ops = list(oracle.gen_dm_random(n, 6, oracle.Rng(seed)))
+    extra = []
+    if n >= 3:
+        g = random_unitary(rng, 2).reshape(-1)
+        extra.append((3, n - 1, 0, g, list(range(n - 1))))
+        extra.append((4, 1, 0, None, None))
+        extra.append((3, 0, 0, gate_set()["x"], [1, 2]))
+    arr = np.concatenate([np.asarray(ops), ops_array(extra)]) if extra else np.asarray(ops)
+    arr = np.concatenate([arr, ops_array([(4, q, q, None, None) for q in range(n)][: max(1, n // 2)])])
+    ks = np.array([random_unitary(rng, 2).reshape(-1) * np.sqrt(0.5) for _ in range(2)])
+    for noise in (None, {"kind_1q": "depolarizing", "p_1q": 0.03, "kind_2q": "amplitude_damping", "p_2q": 0.05},
+                  {"custom_1q": ks, "custom_2q": ks, "readout_error": 0.2}):
+        for s_ in (0, 5):
+            a, ba = oracle.dm_run(arr, n, noise, s_, num_classical=n)
+            b, bb = oracle.dm_run(arr, n, noise, s_, num_classical=n, superop=True)
+            assert ba == bb, (noise, s_)
+            assert np.max(np.abs(a - b)) < 1e-12, (noise, s_)
+
+
 def test_dm_measure_readout_draw_order():  # density_matrix.rs:155-169
     # |1> measured with readout error 1.0 - eps: the recorded bit flips when the
     # second draw is below readout_error; it is drawn only when readout_error > 0.
