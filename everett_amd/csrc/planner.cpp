@@ -1143,9 +1143,14 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             a_out = ident;
             bool out_pending = false;
             uint64_t out_touched = 0;  // qubits of the relabellings composed into a_out
-            for (int id : srem) {
+            for (size_t si = 0; si < srem.size(); ++si) {
+                const int id = srem[si];
                 const GOp &op = g[id];
                 const uint64_t tq = op.touched();
+                if ((sblocked & Q) == Q) {  // every op touches a tile qubit: the rest all wait
+                    sdefer.insert(sdefer.end(), srem.begin() + ptrdiff_t(si), srem.end());
+                    break;
+                }
                 if (tq & sblocked) {
                     sblocked |= tq;
                     sdefer.push_back(id);

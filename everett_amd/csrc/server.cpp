@@ -1,7 +1,10 @@
 // Continuous-batching circuit server (ev_server_* in include/everett.h).
 //
-// submit() plans the circuit on the caller's thread (fused passes, as for a
-// single run) and queues it.  The scheduler thread:
+// submit() validates the circuit on the caller's thread, stamps the request's
+// submit time (its latency runs from there, planning included) and hands a
+// copy to a few planner threads, which plan it (fused passes, as for a single
+// run) and queue it; a burst of submissions is planned in parallel and never
+// holds up the caller.  The scheduler thread:
 //   1. admits queued requests in arrival order while their statevectors fit
 //      the HBM budget: allocates the state (from a size-keyed cache of device
 //      blocks), uploads the plan on a side stream, resets the state;
@@ -55,6 +58,8 @@ struct Request {
     uint64_t id = 0;
     int n = 0;
     uint64_t shots = 0, seed = 0, gates = 0;
+    std::unique_ptr<Circuit> circuit;  // until planned
+    FusedConfig cfg{};
     Plan plan;
     size_t state_bytes = 0;
     void *psi = nullptr;
@@ -160,7 +165,10 @@ struct ev_server {
     hipEvent_t up_ev = nullptr;
     std::chrono::steady_clock::time_point t0;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_plan;
+    std::deque<std::shared_ptr<ev::Request>> plan_queue;  // submitted, not yet planned
+    size_t planning = 0;                                  // requests submitted and not yet queued or failed
+    std::vector<std::thread> planners;
     std::deque<std::shared_ptr<ev::Request>> queue;
     std::vector<std::shared_ptr<ev::Request>> active;     // scheduler thread only: passes left to launch
     std::vector<std::shared_ptr<ev::Request>> finishing;  // scheduler thread only: sampling queued
@@ -409,6 +417,42 @@ std::vector<std::shared_ptr<Request>> step(ev_server *s)
     return ended;
 }
 
+// Planner thread: plans submitted requests and queues them for admission.
+// On shutdown it plans what was already submitted, then returns.
+void planner(ev_server *s)
+{
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_plan.wait(lk, [&] { return s->stop || !s->plan_queue.empty(); });
+            if (s->plan_queue.empty()) return;
+            r = s->plan_queue.front();
+            s->plan_queue.pop_front();
+        }
+        int code = EV_OK;
+        try {
+            r->plan = plan_circuit(*r->circuit, r->n, r->cfg);
+        } catch (const Error &e) {
+            code = e.code;
+        } catch (const std::bad_alloc &) {
+            code = EV_E_OOM;
+        }
+        r->circuit.reset();
+        if (code != EV_OK) fail(s, r, code);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (code == EV_OK) {
+                r->info.passes = r->plan.passes.size();
+                s->queue.push_back(r);
+                s->st.queued = s->queue.size();
+            }
+            --s->planning;
+        }
+        s->cv_work.notify_all();
+    }
+}
+
 void scheduler(ev_server *s)
 {
     for (;;) {
@@ -416,10 +460,12 @@ void scheduler(ev_server *s)
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
-                return s->stop || !s->queue.empty() || !s->active.empty() || !s->finishing.empty();
+                return (s->stop && s->planning == 0) || !s->queue.empty() || !s->active.empty() ||
+                       !s->finishing.empty();
             });
-            if (s->stop && s->queue.empty() && s->active.empty() && s->finishing.empty()) return;
-            // Admission in arrival order while the states fit the budget.
+            if (s->stop && s->planning == 0 && s->queue.empty() && s->active.empty() && s->finishing.empty())
+                return;
+            // Admission in the order planning completes, while the states fit the budget.
             while (!s->queue.empty()) {
                 auto &r = s->queue.front();
                 if (s->cfg.max_active && s->active.size() + admitted.size() >= s->cfg.max_active) break;
@@ -508,6 +554,8 @@ ev_server *server_create(const ev_server_config &cfg)
         throw;
     }
     s->sched = std::thread(scheduler, s);
+    const unsigned np = std::max(1u, std::min(4u, std::thread::hardware_concurrency() / 2));
+    for (unsigned k = 0; k < np; ++k) s->planners.emplace_back(planner, s);
     return s;
 }
 
@@ -518,6 +566,9 @@ void server_destroy(ev_server *s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
     }
+    s->cv_plan.notify_all();
+    for (auto &t : s->planners)
+        if (t.joinable()) t.join();
     s->cv_work.notify_all();
     if (s->sched.joinable()) s->sched.join();
     (void)hipStreamSynchronize(s->stream);
@@ -538,6 +589,7 @@ void server_destroy(ev_server *s)
 
 uint64_t server_submit(ev_server *s, const Circuit &c, uint64_t shots, uint64_t seed)
 {
+    const double t_submit = ms_since(s->t0);
     c.validate();
     const int n = int(c.num_qubits());
     EV_REQUIRE(n >= fused::kMinTile && n <= 30, EV_E_LIMIT, "served circuits need 8..30 qubits");
@@ -556,21 +608,23 @@ uint64_t server_submit(ev_server *s, const Circuit &c, uint64_t shots, uint64_t 
     o.precision = s->cfg.precision;
     o.tile_qubits = s->cfg.tile_qubits;
     o.max_gates_per_pass = s->cfg.max_gates_per_pass;
-    r->plan = plan_circuit(c, n, fused_config(n, s->c64, o));  // on the caller's thread
-    std::lock_guard<std::mutex> lk(s->mu);
-    EV_REQUIRE(!s->stop, EV_E_STATE, "server is shutting down");
-    r->id = s->next_id++;
-    r->info.id = r->id;
-    r->info.num_qubits = uint64_t(n);
-    r->info.gates = r->gates;
-    r->info.passes = r->plan.passes.size();
-    r->info.shots = shots;
-    r->info.submit_ms = ms_since(s->t0);
-    s->reqs[r->id] = r;
-    s->queue.push_back(r);
-    ++s->st.submitted;
-    s->st.queued = s->queue.size();
-    s->cv_work.notify_all();
+    r->cfg = fused_config(n, s->c64, o);  // a bad configuration fails here, on the caller's thread
+    r->circuit = std::make_unique<Circuit>(c);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        EV_REQUIRE(!s->stop, EV_E_STATE, "server is shutting down");
+        r->id = s->next_id++;
+        r->info.id = r->id;
+        r->info.num_qubits = uint64_t(n);
+        r->info.gates = r->gates;
+        r->info.shots = shots;
+        r->info.submit_ms = t_submit;
+        s->reqs[r->id] = r;
+        s->plan_queue.push_back(r);
+        ++s->planning;
+        ++s->st.submitted;
+    }
+    s->cv_plan.notify_one();
     return r->id;
 }
 

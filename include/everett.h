@@ -329,7 +329,8 @@ typedef struct {
 } ev_server_config;
 typedef struct {
     uint64_t id, num_qubits, gates, passes, shots;
-    double submit_ms, admit_ms, done_ms; /* server clock (ms since creation) */
+    double submit_ms, admit_ms, done_ms; /* server clock (ms since creation); submit_ms is taken when
+                                            ev_server_submit is entered, before the circuit is planned */
     int32_t status;                      /* EV_OK, or the error that ended the request */
     int32_t pad;
 } ev_request_info;
@@ -340,7 +341,9 @@ typedef struct {
 } ev_server_stats;
 int ev_server_create(const ev_server_config *cfg, ev_server **out);
 void ev_server_destroy(ev_server *s);   /* drains in-flight work, then frees */
-/* Copies the circuit; returns EV_E_ADMISSION if it can never fit the budget. */
+/* Validates and copies the circuit (planner threads plan the copy); returns
+ * EV_E_ADMISSION if it can never fit the budget.  A planning failure ends the
+ * request with that status. */
 int ev_server_submit(ev_server *s, const ev_circuit *c, uint64_t shots, uint64_t seed, uint64_t *request_id);
 /* Serving wire format: parses OpenQASM 3 text as ev_circuit_from_qasm does
  * and submits the circuit as ev_server_submit does; a parse error returns
