@@ -38,6 +38,8 @@ HBM_PEAK_GBPS = 8000.0
 # build compiles without, so its FP32 vector rate is the unpacked one, equal to FP64's
 VALU_PEAK_TFLOPS = {"c128": 78.6, "c64": 78.6}
 VALU_BOUND = {"c128": "valu-fp64", "c64": "valu-fp32 (unpacked)"}
+# multiply-add rates measured on this GPU at start-up (ev.compute_rates), per precision
+MEASURED_RATES: dict = {}
 HOST_THREADS = min(16, os.cpu_count() or 1)  # the host share of one GPU on the GPU box
 
 
@@ -202,9 +204,13 @@ def compute_roofline(agg, pass_flops, precision):
     key = "c64" if precision == "c64" else "c128"
     peak = VALU_PEAK_TFLOPS[key]
     achieved = per_launch / (avg_ms * 1e-3) / 1e12
-    return {"bound": VALU_BOUND[key], "kernel": "fused_pass",
-            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "flops_per_launch": per_launch}
+    out = {"bound": VALU_BOUND[key], "kernel": "fused_pass",
+           "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+           "flops_per_launch": per_launch}
+    if key in MEASURED_RATES:  # against what multiply-add chains reach on this GPU, as frac_of_copy is for HBM
+        att = MEASURED_RATES[key]["valu_tflops"]
+        out.update({"attainable": att, "frac_of_attainable": achieved / att})
+    return out
 
 
 def bench_sv(name, circuit, n, desc, args, steps, warmup):
@@ -483,6 +489,16 @@ def main() -> int:
     copy_gbps = ev.copy_bandwidth(4 << 30, 10)
     out["copy_peak_gbps"] = copy_gbps
     log(f"streaming copy: {copy_gbps:.0f} GB/s")
+    for p in ("c128", "c64"):
+        MEASURED_RATES[p] = ev.compute_rates(p)
+    out["compute_peaks_tflops"] = {
+        "c128": MEASURED_RATES["c128"], "c64": MEASURED_RATES["c64"],
+        "note": "independent multiply-add chains on this GPU (f64 for c128, f32 for c64): the vector ALU alone, "
+                "the matrix cores alone (v_mfma_f64_4x4x4f64 / v_mfma_f32_4x4x1f32), and both at equal flops; "
+                "a mixed rate no higher than the faster unit alone means the two share one budget"}
+    for p, r in MEASURED_RATES.items():
+        log(f"{p} multiply-add rates: vector {r['valu_tflops']:.1f}, matrix {r['mfma_tflops']:.1f}, "
+            f"both {r['mixed_tflops']:.1f} TFLOP/s")
     if wl in ("all", "c1"):
         c1 = ev.workloads.config1(n1, 40)
         if not args.plain and not args.no_autotune and (args.retune or ev.tune_get(n1, args.precision) is None):

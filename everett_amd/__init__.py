@@ -17,6 +17,7 @@ from .backend import (
     RunStats,
     StateVectorBackend,
     autotune,
+    compute_rates,
     copy_bandwidth,
     last_kernel_stats,
     last_run_stats,
@@ -47,7 +48,7 @@ from .state import State, measure_qubit
 __all__ = [
     "algorithms", "kernel", "workloads", "Batch", "Server", "Circuit", "Gate1", "Gate2", "State", "measure_qubit", "Rng",
     "StateVectorBackend", "Execution", "DensityMatrixBackend", "DensityMatrix", "DensityMatrixExecution",
-    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "copy_bandwidth", "EverettError", "QasmError",
+    "NoiseModel", "Channel", "KrausOp", "RunStats", "last_run_stats", "last_kernel_stats", "run_into", "autotune", "tune_get", "tune_set", "copy_bandwidth", "compute_rates", "EverettError", "QasmError",
     "QubitOutOfRange",
     "ClassicalBitOutOfRange", "DuplicateQubit", "DimensionMismatch", "InvalidArgument", "DeviceError",
     "OutOfMemory", "AdmissionError", "LimitError", "StateError",

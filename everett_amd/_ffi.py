@@ -163,6 +163,10 @@ class EvTuneEntry(C.Structure):
                 ("pad", C.c_uint32), ("ms_per_gate", C.c_double)]
 
 
+class EvComputeRates(C.Structure):
+    _fields_ = [("valu_tflops", C.c_double), ("mfma_tflops", C.c_double), ("mixed_tflops", C.c_double)]
+
+
 class EvPlanInfo(C.Structure):
     _fields_ = [
         ("num_passes", C.c_uint32), ("num_stages", C.c_uint32), ("num_ops", C.c_uint32), ("num_coef", C.c_uint32),
@@ -250,6 +254,7 @@ PROTOTYPES = {
     "ev_batch_compare": (I, [P, P, P, P]),
     "ev_batch_get_stats": (I, [P, C.POINTER(EvBatchStats)]),
     "ev_copy_bandwidth": (I, [SZ, C.c_uint32, C.POINTER(D)]),
+    "ev_compute_rates_measure": (I, [I, C.POINTER(EvComputeRates)]),
     "ev_autotune": (I, [U64, I, C.c_uint32, C.POINTER(EvTuneEntry)]),
     "ev_tune_get": (I, [U64, I, C.POINTER(EvTuneEntry), C.POINTER(I)]),
     "ev_tune_set": (I, [U64, I, C.POINTER(EvTuneEntry)]),

@@ -251,6 +251,15 @@ def tune_set(n: int, precision, entry: dict) -> None:
     check(_ffi.lib().ev_tune_set(n, precision_code(precision), C.byref(e)))
 
 
+def compute_rates(precision="c128") -> dict:
+    """Measured multiply-add rates of this GPU (TFLOP/s) in the arithmetic of
+    `precision` (f64 for c128, f32 for c64): the vector ALU alone, the matrix
+    cores alone, and both at equal flops."""
+    out = _ffi.EvComputeRates()
+    check(_ffi.lib().ev_compute_rates_measure(precision_code(precision), C.byref(out)))
+    return {"valu_tflops": out.valu_tflops, "mfma_tflops": out.mfma_tflops, "mixed_tflops": out.mixed_tflops}
+
+
 def copy_bandwidth(nbytes: int = 4 << 30, reps: int = 10) -> float:
     """Measured streaming-copy bandwidth (read + write GB/s) of this GPU."""
     out = C.c_double()

@@ -907,6 +907,38 @@ int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e)
     });
 }
 
+int ev_compute_rates_measure(ev_precision p, ev_compute_rates *out)
+{
+    return guard([&] {
+        NONNULL(out);
+        const bool f32 = prec_of(p) == ev::kern::C64;
+        void *scr = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (st) (void)hipStreamSynchronize(st);
+            if (scr) (void)hipFree(scr);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        try {
+            EV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            EV_HIP(hipMalloc(&scr, ev::kern::compute_rates_scratch_bytes()));
+            EV_HIP(hipEventCreate(&e0));
+            EV_HIP(hipEventCreate(&e1));
+            const ev::kern::ComputeRates r = ev::kern::compute_rates(f32, scr, st, e0, e1);
+            out->valu_tflops = r.valu_tflops;
+            out->mfma_tflops = r.mfma_tflops;
+            out->mixed_tflops = r.mixed_tflops;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 int ev_copy_bandwidth(size_t bytes, uint32_t reps, double *gbps)
 {
     return guard([&] {

@@ -50,6 +50,15 @@ void normalize(void *psi, Prec p, int n, double *partials, double *norm_slot, hi
 // Streaming copy of `bytes` (a multiple of 32), the HBM bandwidth reference.
 void copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 
+// Measured multiply-add rates in TFLOP/s (rate_kernels.hip): vector ALU
+// alone, matrix cores alone, both at equal flops.  `scratch` holds
+// compute_rates_scratch_bytes(); synchronises `st`.
+struct ComputeRates {
+    double valu_tflops = 0, mfma_tflops = 0, mixed_tflops = 0;
+};
+size_t compute_rates_scratch_bytes();
+ComputeRates compute_rates(bool f32, void *scratch, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+
 // SWAPs of the qubit pairs (a[k], b[k]) (pairwise disjoint) on a 2^n state,
 // in place in one pass: amplitude x trades places with pi(x), pi exchanging
 // the bits of every pair.

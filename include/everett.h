@@ -381,6 +381,22 @@ int ev_tune_set(uint64_t n, ev_precision p, const ev_tune_entry *e);
  * the roofline). */
 int ev_copy_bandwidth(size_t bytes, uint32_t reps, double *gbps);
 
+/* Measured arithmetic rates of this GPU in a precision's arithmetic (f64 for
+ * EV_C128, f32 for EV_C64), in TFLOP/s with a multiply-add counted as 2
+ * flops: independent vector-ALU multiply-add chains alone, matrix-core
+ * accumulations alone (v_mfma_f64_4x4x4f64 / v_mfma_f32_4x4x1f32, the shapes
+ * a 4x4 gate maps onto), and both interleaved at equal flops.  A mixed rate
+ * near the larger of the other two means the units share one budget; near
+ * their sum, that they run side by side.  The achievable-peak reference for
+ * the compute roofline, as ev_copy_bandwidth is for HBM.  No counterpart in
+ * the reference. */
+typedef struct {
+    double valu_tflops;
+    double mfma_tflops;
+    double mixed_tflops;
+} ev_compute_rates;
+int ev_compute_rates_measure(ev_precision p, ev_compute_rates *out);
+
 /* ---------------------------------------------------------- introspection */
 
 /* The fused pass plan ev_sv_run would execute for a measurement-free circuit

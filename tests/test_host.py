@@ -43,6 +43,7 @@ STRUCT_MIRRORS = {  # C struct in include/everett.h -> its ctypes mirror in ever
     "ev_noise_model": "EvNoiseModel", "ev_run_stats": "EvRunStats", "ev_kernel_stat": "EvKernelStat",
     "ev_batch_stats": "EvBatchStats", "ev_server_config": "EvServerConfig", "ev_request_info": "EvRequestInfo",
     "ev_server_stats": "EvServerStats", "ev_tune_entry": "EvTuneEntry", "ev_plan_info": "EvPlanInfo",
+    "ev_compute_rates": "EvComputeRates",
 }
 
 
