@@ -1143,6 +1143,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
             a_out = ident;
             bool out_pending = false;
             uint64_t out_touched = 0;  // qubits of the relabellings composed into a_out
+            uint64_t stouched = 0;     // qubits of the ops this stage takes
             for (size_t si = 0; si < srem.size(); ++si) {
                 const int id = srem[si];
                 const GOp &op = g[id];
@@ -1200,10 +1201,37 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                     sblocked |= affine ? (uint64_t(1) << op.q0) : tq;
                     continue;
                 }
+                if (op.kind == G_U1 && op.cmask == 0 && !(stouched & tq)) {
+                    // A one-qubit op that would open its qubit in this stage, and
+                    // whose next op on that qubit is a pair op of this pass that
+                    // cannot join this stage, waits for the stage that takes the pair:
+                    // taken now, it would run alone or as half of a Kronecker 4x4, and
+                    // the pair's block would cost a second 4x4.  (Past the first stage
+                    // the earliest op left is never held back: nothing is blocked and
+                    // the registers are empty when it is examined.)
+                    const uint64_t qb = uint64_t(1) << op.q0;
+                    bool wait = false;
+                    for (size_t sj = si + 1; sj < srem.size(); ++sj) {
+                        const GOp &nx = g[srem[sj]];
+                        if (!(nx.touched() & qb)) continue;
+                        const uint64_t pm = nx.want() & ~qb;
+                        if (popc(nx.want()) == 2 && popc(pm) == 1 && (pm & Q) == pm) {
+                            const uint32_t both = lmask(qb | pm);
+                            wait = (pm & sblocked) != 0 || (first && (both & lowc)) || popc(S | both) > kRegBits;
+                        }
+                        break;
+                    }
+                    if (wait) {
+                        sblocked |= tq;
+                        sdefer.push_back(id);
+                        continue;
+                    }
+                }
                 uint32_t need = wantm;
                 if (!fits_want) need = lmask(op.need());
                 if (popc(S | need) <= kRegBits && !(first && (need & lowc))) {
                     S |= need;
+                    stouched |= tq;
                     stake.push_back(id);
                 } else {
                     sblocked |= tq;
