@@ -40,6 +40,15 @@ inline dim3 grid_for(uint64_t work)
     if (b > kMaxGrid) b = kMaxGrid;
     return dim3(static_cast<unsigned>(b));
 }
+// The same for blocks of `bs` threads (a power of two dividing the work).
+inline dim3 grid_blocks(uint64_t work, uint64_t bs)
+{
+    uint64_t b = work / bs;
+    const uint64_t cap = (uint64_t(1) << 31) / bs;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return dim3(static_cast<unsigned>(b));
+}
 
 #define GRID_STRIDE(i, count)                                                               \
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (count);         \
@@ -146,29 +155,81 @@ template <class R> __global__ void k_apply1_dense(Amp<R> *psi, uint64_t npairs, 
     }
 }
 
-// Diagonal gate: with m01 = m10 = 0 the reference's sums reduce exactly to
-// m00*a0 and m11*a1 (the zero products contribute signed zeros only).
-template <class R> __global__ void k_apply1_diag(Amp<R> *psi, uint64_t npairs, int k, Amp<R> d0, Amp<R> d1)
+// Workgroup barrier ordering LDS only (a full __syncthreads fence would also
+// wait for the thread's outstanding global stores).
+__device__ __forceinline__ void lds_sync()
 {
-    const uint64_t bit = uint64_t(1) << k;
-    GRID_STRIDE_GROUPS(i0, S, npairs)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The amplitude that lane (this lane XOR m) of the wave holds, m < 64: a
+// cross-lane shuffle, no memory.
+template <class V> __device__ __forceinline__ V lane_xor(V v, int m)
+{
+    V o;
+    o.x = __shfl_xor(v.x, m, 64);
+    o.y = __shfl_xor(v.y, m, 64);
+    return o;
+}
+
+// Target qubit below the wave (k < 6): a thread owns one amplitude x and
+// takes its partner x ^ 2^k from lane (lane ^ 2^k) of its wave with a
+// shuffle, so the launch is one sequential stream in which every load and
+// store moves whole lines.  One thread per pair instead touches each line
+// with two half-used accesses below the 128-byte line (69.6-71.0% of 8 TB/s
+// on positions 0-2 at 30 qubits c128) and reads two streams 2^k apart above
+// it (77.5%); this form measured 82.1%.  The thread computes only its own
+// output, in the reference's operand order (new0 from a0 = its own, new1
+// from a0 = the partner's).  The grid-stride bound is uniform across a wave
+// because len is a multiple of 64 and every wave covers 64 consecutive,
+// 64-aligned amplitudes.
+constexpr int kLaneQubits = 6;
+template <class R> __global__ void k_apply1_dense_lane(Amp<R> *psi, uint64_t len, int k, M2<R> g)
+{
+    const int m = 1 << k;
+    STREAM(x, len)
     {
-        uint64_t j[kU];
-        Amp<R> a0[kU], a1[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            j[u] = insert0(i0 + u * S, k);
-            if (i0 + u * S < npairs) {
-                a0[u] = ldnt(psi + j[u]);
-                a1[u] = ldnt(psi + (j[u] | bit));
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-            if (i0 + u * S < npairs) {
-                stnt(psi + j[u], cmul(d0, a0[u]));
-                stnt(psi + (j[u] | bit), cmul(d1, a1[u]));
-            }
+        const Amp<R> a = ldnt(psi + x);
+        const Amp<R> p = lane_xor(a, m);
+        const Amp<R> o = ((x >> k) & 1) ? cadd(cmul(g.m[2], p), cmul(g.m[3], a)) : cadd(cmul(g.m[0], a), cmul(g.m[1], p));
+        stnt(psi + x, o);
+    }
+}
+
+// Target qubit above the wave but inside a workgroup of BS threads (6 <= k <
+// log2 BS): the same one sequential stream, with the partner taken through
+// LDS (each thread writes its amplitude, reads lane tid ^ 2^k after an
+// LDS-only barrier, and a second barrier keeps the next round's writes
+// behind every read).  BS divides len, so every thread of a block runs the
+// same rounds and meets every barrier.
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_apply1_dense_block(Amp<R> *psi, uint64_t len, int k, M2<R> g)
+{
+    __shared__ Amp<R> buf[BS];
+    const uint32_t m = 1u << k, t = threadIdx.x;
+    for (uint64_t x = xcd_block() * BS + t; x < len; x += uint64_t(gridDim.x) * BS) {
+        const Amp<R> a = ldnt(psi + x);
+        buf[t] = a;
+        lds_sync();
+        const Amp<R> p = buf[t ^ m];
+        lds_sync();
+        const Amp<R> o = ((x >> k) & 1) ? cadd(cmul(g.m[2], p), cmul(g.m[3], a)) : cadd(cmul(g.m[0], a), cmul(g.m[1], p));
+        stnt(psi + x, o);
+    }
+}
+
+// Diagonal gate, any target: with m01 = m10 = 0 the reference's sums reduce
+// exactly to m00*a0 and m11*a1 (the zero products contribute signed zeros
+// only), so a thread owns one amplitude and needs no partner, and the launch
+// is one sequential stream for every target qubit.
+template <class R> __global__ void k_apply1_diag_lane(Amp<R> *psi, uint64_t len, int k, Amp<R> d0, Amp<R> d1)
+{
+    STREAM(x, len)
+    {
+        const Amp<R> a = ldnt(psi + x);
+        stnt(psi + x, cmul(((x >> k) & 1) ? d1 : d0, a));
     }
 }
 
@@ -560,10 +621,21 @@ template <class R> void apply1_t(void *psi, int n, int k, const c128 m[4], hipSt
                                k, g.m[3]);
         break;
     case K1::Diag:
-        hipLaunchKernelGGL(k_apply1_diag<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g.m[0], g.m[3]);
+        hipLaunchKernelGGL(k_apply1_diag_lane<R>, grid_for(2 * npairs), dim3(kBlock), 0, st, A<R>(psi), 2 * npairs, k,
+                           g.m[0], g.m[3]);
         break;
     case K1::Dense:
-        hipLaunchKernelGGL(k_apply1_dense<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g);
+        if (k < kLaneQubits && n >= kLaneQubits)
+            hipLaunchKernelGGL(k_apply1_dense_lane<R>, grid_for(2 * npairs), dim3(kBlock), 0, st, A<R>(psi), 2 * npairs, k,
+                               g);
+        else if (k < 8 && n >= 8)
+            hipLaunchKernelGGL((k_apply1_dense_block<R, 256>), grid_for(2 * npairs), dim3(256), 0, st, A<R>(psi),
+                               2 * npairs, k, g);
+        else if (k < 10 && n >= 10)
+            hipLaunchKernelGGL((k_apply1_dense_block<R, 1024>), grid_blocks(2 * npairs, 1024), dim3(1024), 0, st,
+                               A<R>(psi), 2 * npairs, k, g);
+        else
+            hipLaunchKernelGGL(k_apply1_dense<R>, grid_for(npairs), dim3(kBlock), 0, st, A<R>(psi), npairs, k, g);
         break;
     }
 }
