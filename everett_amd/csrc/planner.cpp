@@ -956,6 +956,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
 {
     Plan P;
     std::vector<c128> pool = pool_in;  // grows with fused coefficients
+    std::vector<int> next_same(g.size(), -1);  // per pass: see the stage loop
     const uint64_t all = (n >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
     const uint64_t lowmask = (uint64_t(1) << low) - 1;
     std::vector<int> rem(g.size());
@@ -965,13 +966,17 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
     // fit the tile and it does not depend on a deferred op.  `grow` lets the
     // tile take new qubits up to t (program-order greedy); otherwise the tile
     // is the fixed set Q0.
+    // defer holds the ops deferred before the scan stopped at rem[stop]; the
+    // rest of rem follows them (appended for the chosen tile only: copying it
+    // for every candidate cost more than planning itself).
     struct Formed {
         uint64_t Q;
         float work;
         std::vector<int> take, defer;
+        size_t stop;
     };
     auto form = [&](uint64_t Q0, bool grow) {
-        Formed f{Q0, 0.f, {}, {}};
+        Formed f{Q0, 0.f, {}, {}, 0};
         uint64_t blocked = 0;
         float cost = 0;
         size_t i = 0;
@@ -999,7 +1004,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 break;
             }
         }
-        for (; i < rem.size(); ++i) f.defer.push_back(rem[i]);
+        f.stop = i;
         f.work = cost + 0.01f * float(f.take.size());
         return f;
     };
@@ -1021,6 +1026,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         }
         uint64_t Q = best.Q;
         std::vector<int> take = std::move(best.take);
+        best.defer.insert(best.defer.end(), rem.begin() + ptrdiff_t(best.stop), rem.end());
         rem.swap(best.defer);
 
         // Pad Q to exactly t qubits with the lowest unused ones.
@@ -1130,6 +1136,17 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
         // reading through its inverse (relabel tables of the next stage).
         // Later ops on its qubits then wait for the next stage.
         std::vector<int> srem = take;
+        // the next op of the pass on the qubit of each one-qubit op (-1: none)
+        {
+            int lastq[64];
+            std::fill(lastq, lastq + 64, -1);
+            for (size_t a = take.size(); a-- > 0;) {
+                const int id = take[a];
+                const GOp &op = g[size_t(id)];
+                if (op.kind == G_U1 && op.cmask == 0) next_same[size_t(id)] = lastq[op.q0];
+                for (uint64_t m = op.touched(); m; m &= m - 1) lastq[__builtin_ctzll(m)] = id;
+            }
+        }
         uint32_t lastS = 0;
         bool first = true;
         Relabel ident, a_in, a_out;
@@ -1211,15 +1228,13 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                     // the registers are empty when it is examined.)
                     const uint64_t qb = uint64_t(1) << op.q0;
                     bool wait = false;
-                    for (size_t sj = si + 1; sj < srem.size(); ++sj) {
-                        const GOp &nx = g[srem[sj]];
-                        if (!(nx.touched() & qb)) continue;
+                    if (next_same[size_t(id)] >= 0) {  // still ahead in srem: it waits for this op
+                        const GOp &nx = g[size_t(next_same[size_t(id)])];
                         const uint64_t pm = nx.want() & ~qb;
                         if (popc(nx.want()) == 2 && popc(pm) == 1 && (pm & Q) == pm) {
                             const uint32_t both = lmask(qb | pm);
                             wait = (pm & sblocked) != 0 || (first && (both & lowc)) || popc(S | both) > kRegBits;
                         }
-                        break;
                     }
                     if (wait) {
                         sblocked |= tq;
