@@ -685,9 +685,23 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
     one_tile<R, M>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
 }
 
-// Batch kernel: workgroup b runs linear tile b of the round, whose entry
-// (entries sorted by tile_begin) is found by binary search; an entry covers
-// the tiles [tile_off, tile_off + its count) of its pass.
+// XCD-aware order for a run of any length: workgroup b runs on XCD b mod 8,
+// and XCD x takes the contiguous run of tiles that starts at
+// x * (total / 8) + min(x, total mod 8) (the first total mod 8 XCDs take one
+// tile more), so tiles that neighbour in memory share an L2.
+__device__ __forceinline__ uint32_t xcd_linear(uint32_t b, uint32_t total)
+{
+    const uint32_t x = b & 7u, q = total >> 3, rm = total & 7u;
+    return x * q + min(x, rm) + (b >> 3);
+}
+
+// Batch kernel: workgroup b belongs to the entry whose run of workgroups
+// holds it (entries sorted by tile_begin, found by binary search); an entry
+// covers the tiles [tile_off, tile_off + count) of its pass.  Within the run
+// the tiles go to XCDs in contiguous ranges (xcd_linear over the run; the
+// run's workgroups with one residue mod 8 share an XCD), while the entries
+// stay interleaved over every XCD, so entries of unequal cost per tile still
+// load all eight evenly.
 template <class R>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_batch(
     const BatchEntry *__restrict__ entries, int nentries)
@@ -701,8 +715,10 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
         if (ent[mid].tile_begin <= lin) lo = mid; else hi = mid - 1;
     }
     const cptr<BatchEntry> e = ent + lo;
+    const uint64_t end = lo + 1 < nentries ? ent[lo + 1].tile_begin : uint64_t(gridDim.x);
+    const uint32_t local = xcd_linear(uint32_t(lin - e->tile_begin), uint32_t(end - e->tile_begin));
     one_tile<R, false>(tile_ref<R>(static_cast<Amp<R> *>(e->psi), as_const(e->pass), as_const(e->stages), as_const(e->ops),
-                            as_const(static_cast<const Amp<R> *>(e->coef)), lin - e->tile_begin + e->tile_off),
+                            as_const(static_cast<const Amp<R> *>(e->coef)), local + e->tile_off),
                 lds_raw);
 }
 

@@ -30,7 +30,9 @@ struct BatchEntry {
 };
 
 // One launch over every entry of a round: total_tiles workgroups of
-// 2^(max_t-4) threads, workgroup b running linear tile b.
+// 2^(max_t-4) threads.  Entry e owns the workgroups [tile_begin, next
+// entry's tile_begin) and runs one tile of its pass on each, the tiles
+// dealt out in an XCD-aware order within that run.
 void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st);
 
 }  // namespace ev::fused
