@@ -45,6 +45,10 @@ enum OpKind : uint8_t {
     kSwap = 6,   // exchange register bits b0 > b1
     kPhase = 7,  // multiply registers whose bit b0 is set (b0 = 4: every register) and that meet
                  // creg by the lane's product of b1 conditioned phase factors (see DevOp)
+    kU2X = 8,    // 4x4 on register bits b0 > b1 (as kU2) whose only nonzero entries are m00 m03
+                 // m11 m22 m30 m33, six pool entries in that order: |00> and |11> mix, |01> and
+                 // |10> scale.  Density-matrix passes only (the superoperator of a depolarizing or
+                 // amplitude-damping channel on a (row, column) bit pair has this form)
 };
 
 constexpr uint32_t kPhaseAll = 4;     // kPhase b0: the factor applies to all 16 registers

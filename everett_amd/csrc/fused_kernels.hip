@@ -189,6 +189,31 @@ template <class V, int BH, int BL> __device__ __forceinline__ void op_u2(const R
     }
 }
 
+// Sparse 4x4 (kU2X): s = m00 m03 m11 m22 m30 m33 of a gate in the basis
+// |b_hi b_lo> whose other entries are zero, so |00> and |11> mix and |01>
+// and |10> only scale: 6 multiply-adds per amplitude instead of 16.
+template <class V> __device__ __forceinline__ V dot2(V m0, V x0, V m1, V x1)
+{
+    decltype(V::x) ax = -(m0.y * x0.y), ay = m0.y * x0.x;
+    ax = dfma(m1.x, x1.x, dfma(-m1.y, x1.y, ax));
+    ay = dfma(m1.x, x1.y, dfma(m1.y, x1.x, ay));
+    return mk<V>(dfma(m0.x, x0.x, ax), dfma(m0.x, x0.y, ay));
+}
+
+template <class V, int BH, int BL>
+__device__ __forceinline__ void op_u2x(const Regs<V> &in, Regs<V> &out, V s0, V s1, V s2, V s3, V s4, V s5)
+{
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        if (j & ((1 << BH) | (1 << BL))) continue;
+        const int a = j, b = j | (1 << BL), c = j | (1 << BH), d = j | (1 << BH) | (1 << BL);
+        out[a] = dot2(s0, in[a], s1, in[d]);
+        out[b] = cmul(s2, in[b]);
+        out[c] = cmul(s3, in[c]);
+        out[d] = dot2(s4, in[a], s5, in[d]);
+    }
+}
+
 // Complex64 dense 4x4 on the matrix cores: v_mfma_f32_4x4x1_16b_f32 runs 16
 // independent 4x4 outer products, block b = lanes 4b..4b+3, where lane 4b+j
 // holds column j of B (one float) and receives column j of D (four floats),
@@ -351,7 +376,9 @@ template <class V, bool M> __device__ __forceinline__ U2Row u2_row(const OpView<
     return r;
 }
 
-template <class V, bool M>
+// X: the pass may hold kU2X ops (density-matrix passes); other passes run a
+// kernel without that arm.
+template <class V, bool M, bool X = false>
 __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, Regs<V> &out, uint64_t gfix,
                                        cptr<V> coef, const U2Row &row)
 {
@@ -416,6 +443,15 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
         } else {
             EV_CASE_B(op_diagr, in, out, mk<V>(1, 0), f, true, op.creg);
         }
+        break;
+    }
+    case (X ? uint32_t(kU2X) : 0x100u): {  // no such arm (kind < 0x100) unless X
+        // the six coefficients go by value: a local array passed by address
+        // shares a stack slot with the dense arm's row and keeps both in scratch
+        using F = decltype(V::x);
+        const cptr<F> f = (cptr<F>)(coef + op.coef);
+        const auto e = [&](int k) { return mk<V>(f[2 * k], f[2 * k + 1]); };
+        EV_CASE_PAIR(op_u2x, in, out, e(0), e(1), e(2), e(3), e(4), e(5));
         break;
     }
     default: EV_CASE_PAIR(op_swap, in, out); break;
@@ -508,7 +544,7 @@ __device__ __forceinline__ void load_tile(const TileRef<R> &tr, uint32_t tid, Re
 // descriptor loads are in flight while the current op computes); an odd
 // last op is followed by one copy back to v.  (Exchanging straight from w
 // instead keeps both arrays live across the exchange, which spills.)
-template <class V, bool M>
+template <class V, bool M, bool X = false>
 __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V> &w, uint64_t gfix, cptr<DevOp> ops,
                                               cptr<V> coef)
 {
@@ -518,12 +554,12 @@ __device__ __forceinline__ void run_stage_ops(cptr<Stage> st, Regs<V> &v, Regs<V
         const OpView<V> a = fetch_op<V>(ops + o), b = fetch_op<V>(ops + o + 1);
         // matrix-core gates: both rows are in flight before the first op runs
         const U2Row ra = u2_row<V, M>(a, coef), rb = u2_row<V, M>(b, coef);
-        run_op<V, M>(a, v, w, gfix, coef, ra);
-        run_op<V, M>(b, w, v, gfix, coef, rb);
+        run_op<V, M, X>(a, v, w, gfix, coef, ra);
+        run_op<V, M, X>(b, w, v, gfix, coef, rb);
     }
     if (o < oe) {
         const OpView<V> a = fetch_op<V>(ops + o);
-        run_op<V, M>(a, v, w, gfix, coef, u2_row<V, M>(a, coef));
+        run_op<V, M, X>(a, v, w, gfix, coef, u2_row<V, M>(a, coef));
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) v[j] = w[j];
     }
@@ -611,7 +647,7 @@ __device__ __forceinline__ void exchange(Regs<Amp<R>> &v, void *lds, bool active
 // widest tile) run the register arithmetic on don't-care values but skip
 // every memory access, and keep every barrier; the op loop thus stays out of
 // divergent control flow and its descriptor reads stay scalar.
-template <class R, bool M>
+template <class R, bool M, bool X = false>
 __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid, Regs<Amp<R>> &v, void *lds)
 {
     using V = Amp<R>;
@@ -621,7 +657,7 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
     Lane ln = lane_of(tr.stages, tid, tr.base, tr.swz_a, tr.swz_b);
     for (int s = 0; s < nst; ++s) {
         const cptr<Stage> st = tr.stages + s;
-        run_stage_ops<V, M>(st, v, w, ln.gfix, tr.ops, tr.coef);
+        run_stage_ops<V, M, X>(st, v, w, ln.gfix, tr.ops, tr.coef);
         if (s + 1 < nst) {
             const cptr<Stage> nx = st + 1;
             // the global index bits only where an op of the stage tests them or the stage scatters
@@ -655,14 +691,14 @@ __device__ __forceinline__ void compute_tile(const TileRef<R> &tr, uint32_t tid,
 // dispatcher keeps every CU full; a persistent grid that loops over tiles
 // measured 1.7x slower on MI355X (any CU slot it fails to place delays a
 // whole strip of tiles).
-template <class R, bool M>
+template <class R, bool M, bool X = false>
 __device__ __forceinline__ void one_tile(const TileRef<R> &tr, void *lds)
 {
     using V = Amp<R>;
     const uint32_t tid = threadIdx.x;
     Regs<V> v;
     load_tile<R>(tr, tid, v);  // threads beyond the tile read an in-tile alias (their bits above t-4 are ignored)
-    compute_tile<R, M>(tr, tid, v, lds);
+    compute_tile<R, M, X>(tr, tid, v, lds);
 }
 
 // Register budget: 168 VGPRs (3 waves per SIMD; the two c128 register
@@ -675,14 +711,14 @@ template <class R> constexpr int kWavesPerSimd = 3;
 // XCD-aware tile order: dispatch places workgroup b on XCD b mod 8, so XCD x
 // gets the contiguous tile range [x * tiles/8, (x+1) * tiles/8), and tiles
 // that neighbour in memory share an L2.
-template <class R, bool M>
+template <class R, bool M, bool X>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<R>))) void k_fused_pass(
     Amp<R> *__restrict__ psi, const Pass *__restrict__ pass, const Stage *__restrict__ stages,
     const DevOp *__restrict__ ops, const Amp<R> *__restrict__ coef, uint32_t per, uint32_t xcds)
 {
     extern __shared__ unsigned char lds_raw[];
     const uint64_t T = uint64_t(blockIdx.x % xcds) * per + blockIdx.x / xcds;
-    one_tile<R, M>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
+    one_tile<R, M, X>(tile_ref<R>(psi, as_const(pass), as_const(stages), as_const(ops), as_const(coef), T), lds_raw);
 }
 
 // XCD-aware order for a run of any length: workgroup b runs on XCD b mod 8,
@@ -722,7 +758,7 @@ __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(kWa
                 lds_raw);
 }
 
-template <class R, bool M>
+template <class R, bool M, bool X>
 void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops, const void *coef,
                    hipStream_t st)
 {
@@ -730,7 +766,7 @@ void launch_pass_t(void *psi, int n, int t, const Pass *pass, const Stage *stage
     const unsigned block = 1u << (t - kRegBits);  // <= kMaxThreads = the kernel's launch bound
     const size_t lds = (size_t(1) << t) * kLdsSlotBytes<R>;
     const uint32_t xcds = (tiles % 8 == 0) ? 8u : 1u;
-    hipLaunchKernelGGL((k_fused_pass<R, M>), dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
+    hipLaunchKernelGGL((k_fused_pass<R, M, X>), dim3(unsigned(tiles)), dim3(block), lds, st, static_cast<Amp<R> *>(psi), pass,
                        stages, ops, static_cast<const Amp<R> *>(coef), uint32_t(tiles / xcds), xcds);
 }
 
@@ -745,14 +781,20 @@ void launch_batch_t(const BatchEntry *entries, int nentries, uint64_t total_tile
 }  // namespace
 
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
-                 const void *coef, bool mfma_u2, hipStream_t st)
+                 const void *coef, bool mfma_u2, bool sparse_u2, hipStream_t st)
 {
     EV_REQUIRE(t >= kMinTile && t <= kMaxTile && t <= n, EV_E_ARG, "fused pass tile width out of range");
     EV_REQUIRE(n - t < 32, EV_E_LIMIT, "fused pass has more than 2^32 - 1 tiles");
     EV_REQUIRE(c64 || !mfma_u2, EV_E_ARG, "matrix-core dense gates are a complex64 option");
-    if (!c64) launch_pass_t<double, false>(psi, n, t, pass, stages, ops, coef, st);
-    else if (mfma_u2) launch_pass_t<float, true>(psi, n, t, pass, stages, ops, coef, st);
-    else launch_pass_t<float, false>(psi, n, t, pass, stages, ops, coef, st);
+    EV_REQUIRE(!c64 || !sparse_u2, EV_E_ARG, "sparse 4x4 ops are a complex128 (density-matrix) option");
+    if (!c64) {
+        if (sparse_u2) launch_pass_t<double, false, true>(psi, n, t, pass, stages, ops, coef, st);
+        else launch_pass_t<double, false, false>(psi, n, t, pass, stages, ops, coef, st);
+    } else if (mfma_u2) {
+        launch_pass_t<float, true, false>(psi, n, t, pass, stages, ops, coef, st);
+    } else {
+        launch_pass_t<float, false, false>(psi, n, t, pass, stages, ops, coef, st);
+    }
 }
 
 void launch_batch(bool c64, const BatchEntry *entries, int nentries, uint64_t total_tiles, int max_t, hipStream_t st)

@@ -11,9 +11,9 @@ namespace ev::fused {
 // workgroups of 2^(t-4) threads, one tile of 2^t amplitudes each.
 // Descriptor pointers are device memory; coef holds double2 (c128) or float2
 // (c64) entries.  mfma_u2 (c64 only) runs dense 4x4 gates on the matrix
-// cores.
+// cores; sparse_u2 (c128 only) must be set when the pass holds kU2X ops.
 void launch_pass(void *psi, bool c64, int n, int t, const Pass *pass, const Stage *stages, const DevOp *ops,
-                 const void *coef, bool mfma_u2, hipStream_t st);
+                 const void *coef, bool mfma_u2, bool sparse_u2, hipStream_t st);
 
 // One entry of a batched round: a run of consecutive tiles of one state's next
 // pass, from tile tile_off on.  tile_begin is the prefix sum of the tile

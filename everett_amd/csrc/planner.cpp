@@ -627,10 +627,23 @@ class StageFuser {
     std::map<int, int> open_;
 };
 
+// A 4x4 (basis |b_hi b_lo>, row-major) whose entries other than m00 m03 m11
+// m22 m30 m33 are exactly zero: it mixes |00> with |11> and scales |01>
+// and |10> (fused::kU2X).
+bool x_form(const c128 *m)
+{
+    static constexpr int kZero[] = {1, 2, 4, 6, 7, 8, 9, 11, 13, 14};
+    for (int k : kZero)
+        if (m[k].re != 0 || m[k].im != 0) return false;
+    return true;
+}
+
 // Lowers one planner op into a device op for register set S (local
-// positions); regbit maps a global qubit to its register bit or -1.
+// positions); regbit maps a global qubit to its register bit or -1.  With
+// sparse_u2, a 4x4 of the x_form shape becomes a kU2X.
 template <class RegBit>
-DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const RegBit &regbit, std::vector<c128> &coef)
+DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const RegBit &regbit, std::vector<c128> &coef,
+               bool sparse_u2)
 {
     DevOp d{};
     uint32_t kind = 0, b0 = 0, b1 = 0, flags = 0;
@@ -673,20 +686,21 @@ DevOp lower_op(const GOp &op, const std::vector<c128> &pool, uint32_t S, const R
         break;
     }
     case G_U2: {
-        kind = fused::kU2;
         const int ra = regbit(S, op.q0), rb = regbit(S, op.q1);
         const c128 *m = pool.data() + op.coef;
-        if (ra > rb) {
-            b0 = uint32_t(ra);
-            b1 = uint32_t(rb);
-            coef.insert(coef.end(), m, m + 16);
-        } else {  // reorder the basis |q_a q_b> -> |q_b q_a>
-            b0 = uint32_t(rb);
-            b1 = uint32_t(ra);
-            auto sw = [](int x) { return ((x & 1) << 1) | (x >> 1); };
-            c128 mm[16];
-            for (int r = 0; r < 4; ++r)
-                for (int cc = 0; cc < 4; ++cc) mm[sw(r) * 4 + sw(cc)] = m[r * 4 + cc];
+        b0 = uint32_t(std::max(ra, rb));
+        b1 = uint32_t(std::min(ra, rb));
+        // the basis |b0 b1>: as stored when q_a has the higher register bit,
+        // else reordered from |q_a q_b> to |q_b q_a>
+        auto sw = [&](int x) { return ra > rb ? x : ((x & 1) << 1) | (x >> 1); };
+        c128 mm[16];
+        for (int r = 0; r < 4; ++r)
+            for (int cc = 0; cc < 4; ++cc) mm[sw(r) * 4 + sw(cc)] = m[r * 4 + cc];
+        if (sparse_u2 && x_form(mm)) {
+            kind = fused::kU2X;
+            for (int k : {0, 3, 5, 10, 12, 15}) coef.push_back(mm[k]);
+        } else {
+            kind = fused::kU2;
             coef.insert(coef.end(), mm, mm + 16);
         }
         break;
@@ -765,6 +779,7 @@ std::vector<DevOp> merge_phases(const std::vector<DevOp> &in, std::vector<c128> 
         case fused::kCU1:
         case fused::kCX: targets = 1u << b0; break;
         case fused::kU2:
+        case fused::kU2X:
         case fused::kSwap: targets = (1u << b0) | (1u << b1); break;
         default: break;
         }
@@ -849,6 +864,7 @@ double op_flops_per_amp(const DevOp &d)
     case fused::kU1:
     case fused::kCU1: f = 14; break;
     case fused::kU2: f = 30; break;
+    case fused::kU2X: f = 10; break;  // two 2-term dot products (14 each) and two complex scales (6 each) per 4
     case fused::kDiagR: f = (flags & fused::kD0IsOne) ? 3 : 6; break;
     case fused::kDiagT: f = 6; break;
     case fused::kPhase: f = b0 == fused::kPhaseAll ? 6 : 3; break;
@@ -952,7 +968,7 @@ void pass_swizzle(Pass &ps, Stage *st, bool wide64)
 // while pass p + 1 is planned.
 using PassSink = std::function<void(const Plan &, size_t)>;
 Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, int t, int low, float budget,
-          int coalesce, const PassSink &on_pass = {})
+          int coalesce, bool sparse_u2, const PassSink &on_pass = {})
 {
     Plan P;
     std::vector<c128> pool = pool_in;  // grows with fused coefficients
@@ -1112,7 +1128,7 @@ Plan plan(const std::vector<GOp> &g, const std::vector<c128> &pool_in, int n, in
                 st.rroff[j] = image(off);
             }
             std::vector<DevOp> lowered;
-            for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef));
+            for (const GOp &op : fused) lowered.push_back(lower_op(op, pool, S, regbit, P.coef, sparse_u2));
             for (const DevOp &d : merge_phases(lowered, P.coef)) P.devops.push_back(d);
             st.op_end = int(P.devops.size());
             for (int o = st.op_begin; o < st.op_end; ++o) {
@@ -1400,9 +1416,11 @@ class PassStreamer {
         }
         DevOp *ho = reinterpret_cast<DevOp *>(host + o_ops);
         double fpa = 0;
+        bool sparse = false;  // the pass needs the kernel with the kU2X arm
         for (size_t i = 0; i < nops; ++i) {
             const DevOp &d = P.devops[size_t(op0) + i];
             fpa += op_flops_per_amp(d);
+            sparse = sparse || (d.code & 0xff) == fused::kU2X;
             ho[i] = d;
             ho[i].coef = d.coef - uint32_t(coef0);  // every op of the pass was lowered after coef0
         }
@@ -1420,7 +1438,7 @@ class PassStreamer {
         prof_.begin();
         fused::launch_pass(s_->psi, c64, s_->n, ps.t, reinterpret_cast<const Pass *>(dev),
                            reinterpret_cast<const Stage *>(dev + o_stage), reinterpret_cast<const DevOp *>(dev + o_ops),
-                           dev + o_coef, mfma_, s_->stream);
+                           dev + o_coef, mfma_, sparse, s_->stream);
         prof_.end("fused_pass", full_pass_bytes(s_->n, ab));
         st_.pass_flops += fpa * std::ldexp(1.0, s_->n);
         ++st_.passes;
@@ -1504,7 +1522,7 @@ void flush_segment(ev_sv *s, Lowering &low, const FusedConfig &cfg, RunStats &st
     if (!low.ops.empty()) {
         PassStreamer launch(s, cfg.mfma_u2, st, prof);
         double in_sink = 0;  // host time spent uploading and launching, not planning
-        plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce, [&](const Plan &P, size_t p) {
+        plan(low.ops, low.pool, s->n, cfg.t, cfg.low, budget, cfg.coalesce, cfg.sparse_u2, [&](const Plan &P, size_t p) {
             const double a = now_ms();
             launch(P, p);
             in_sink += now_ms() - a;
@@ -1585,7 +1603,7 @@ Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg, bool peel_swa
     std::vector<std::pair<int, int>> tail;
     if (peel_swaps) tail = peel_swap_tail(low, cfg, n);
     const float budget = (cfg.t >= n) ? 1e30f : cfg.budget;
-    Plan P = low.ops.empty() ? Plan{} : plan(low.ops, low.pool, n, cfg.t, cfg.low, budget, cfg.coalesce);
+    Plan P = low.ops.empty() ? Plan{} : plan(low.ops, low.pool, n, cfg.t, cfg.low, budget, cfg.coalesce, cfg.sparse_u2);
     P.swap_tail = std::move(tail);
     return P;
 }
@@ -1637,7 +1655,8 @@ void execute_planned_dm(ev_dm *d, const Circuit &c, const DmNoise &nz, Rng &rng,
         dopt.low_qubits = 2;
         dopt.max_gates_per_pass = 1u << 30;
     }
-    const FusedConfig cfg = fused_config(2 * n, false, dopt);
+    FusedConfig cfg = fused_config(2 * n, false, dopt);
+    cfg.sparse_u2 = true;  // channel superoperators on a (row, column) pair mix only |00> and |11>
     Lowering low(2 * n);
     size_t i = 0;
     while (i < c.size()) {

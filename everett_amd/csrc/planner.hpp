@@ -24,6 +24,7 @@ struct FusedConfig {
     float budget;
     int coalesce;
     bool mfma_u2;  // complex64: dense 4x4 gates on the matrix cores
+    bool sparse_u2;  // lower 4x4s of the kU2X form to kU2X (density-matrix passes)
 };
 FusedConfig fused_config(int n, bool c64, const ev_run_options &opt);
 
