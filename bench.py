@@ -338,7 +338,25 @@ def bench_c3(args) -> dict:
                      "avg_launch_ms": rounds_ms / max(1, st["rounds"]), "launches": st["rounds"],
                      "note": "algorithmic bytes of all rounds (every pass reads and writes its state once) over the "
                              "HIP-event time of the rounds"},
+        "compute_roofline": batch_compute_roofline(st["pass_flops"], rounds_ms, st["rounds"]),
     }
+
+
+def batch_compute_roofline(flops, rounds_ms, rounds):
+    """The batched rounds against the vector ALU's FP32 rate: algorithmic flops of
+    every pass (the planner's count per device op) over the HIP-event time of the
+    rounds.  The states are at most 22 qubits (32 MB), so the rounds are bound
+    by arithmetic, not by HBM."""
+    if not rounds_ms or not flops:
+        return None
+    achieved = flops / (rounds_ms * 1e-3) / 1e12
+    peak = VALU_PEAK_TFLOPS["c64"]
+    out = {"bound": VALU_BOUND["c64"], "kernel": "fused_batch", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+           "frac": achieved / peak, "flops": flops, "avg_launch_ms": rounds_ms / max(1, rounds), "launches": rounds}
+    if "c64" in MEASURED_RATES:
+        att = MEASURED_RATES["c64"]["valu_tflops"]
+        out.update({"attainable": att, "frac_of_attainable": achieved / att})
+    return out
 
 
 # ----------------------------------------------------------------------- c4

@@ -138,6 +138,7 @@ class EvBatchStats(C.Structure):
         ("circuits", C.c_uint64), ("logical_gates", C.c_uint64), ("rounds", C.c_uint64), ("passes", C.c_uint64),
         ("tiles", C.c_uint64), ("plan_ms", C.c_double), ("upload_ms", C.c_double), ("device_ms", C.c_double),
         ("renorm_ms", C.c_double), ("graph", C.c_uint32), ("groups", C.c_uint32), ("pass_bytes", C.c_double),
+        ("pass_flops", C.c_double),
     ]
 
 

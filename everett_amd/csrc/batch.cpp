@@ -224,6 +224,7 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
                 max_passes = std::max(max_passes, loc[k].npass);
                 b->stats.passes += P.passes.size();
                 b->stats.pass_bytes += double(P.passes.size()) * 2.0 * double(uint64_t(1) << b->n[i0 + k]) * double(ab);
+                b->stats.pass_flops += plan_flops_per_amp(P) * double(uint64_t(1) << b->n[i0 + k]);
             }
             const size_t entries_off = off;
             std::vector<size_t> round_off(max_passes), round_n(max_passes);

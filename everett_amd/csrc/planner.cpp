@@ -1586,6 +1586,13 @@ void execute_planned(ev_sv *s, const Circuit &c, Rng &rng, uint8_t *classical, c
     flush_segment(s, low, cfg, st, prof);
 }
 
+double plan_flops_per_amp(const Plan &P)
+{
+    double f = 0;
+    for (const DevOp &d : P.devops) f += op_flops_per_amp(d);
+    return f;
+}
+
 Plan plan_circuit(const Circuit &c, int n, const FusedConfig &cfg, bool peel_swaps)
 {
     EV_REQUIRE(n >= fused::kMinTile, EV_E_ARG, "fused plans need at least 8 qubits");

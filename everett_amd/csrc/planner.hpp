@@ -57,4 +57,8 @@ struct PlanBlob {
 };
 PlanBlob debug_plan(const Circuit &c, const ev_run_options &opt);
 
+// Algorithmic floating-point operations per state amplitude of all of a
+// plan's device ops (each pass applies its ops to every amplitude once).
+double plan_flops_per_amp(const Plan &P);
+
 }  // namespace ev

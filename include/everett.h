@@ -292,6 +292,7 @@ typedef struct {
     uint32_t groups; /* pipelined groups: the host plans group g+1 while the device runs group g */
     double pass_bytes; /* algorithmic HBM bytes of all fused passes: per pass, every amplitude of its
                           state read and written once (2 * 2^n * amplitude size) */
+    double pass_flops; /* algorithmic floating-point operations of all fused passes (an FMA counts 2) */
 } ev_batch_stats;
 int ev_batch_run(const ev_circuit *const *circuits, size_t count, const ev_run_options *opt, ev_batch **out,
                  ev_error *err);

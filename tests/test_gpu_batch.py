@@ -65,6 +65,12 @@ def test_batch_uneven_sizes_like_config3():
     i = int(np.argmin([c.num_qubits() for c in cs]))
     want, _ = oracle.sv_run(cs[i].ops(), cs[i].num_qubits())
     assert np.max(np.abs(ref.amplitudes(i) - want)) < 1e-10
+    # the batch's algorithmic flops are the single runs' (same plans, summed per pass)
+    flops = 0.0
+    for c in cs:
+        ev.StateVectorBackend.run(c, precision="c64")
+        flops += ev.last_run_stats().pass_flops
+    assert x.stats()["pass_flops"] == pytest.approx(flops, rel=1e-12)
 
 
 def test_batch_rejects_measurement():
