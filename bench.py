@@ -105,6 +105,32 @@ def cpu_baseline_all_cores(circuit, n: int, seconds: float) -> dict:
                       f"OpenMP on {HOST_THREADS} threads"}
 
 
+def c0_cpu_comparison(circuit, gpu_gates_per_s: float, seconds: float) -> dict:
+    """c0 beside the CPU: the oracle on 1 thread and on all host cores (gate prefixes of the same
+    20-qubit circuit, bounded in time), and the GPU's fused result against the whole circuit run by the
+    oracle (all host cores: the gate loops have no reductions, so it is the 1-thread result bit for bit)."""
+    import numpy as np
+
+    import everett_amd as ev
+    import oracle
+
+    n = circuit.num_qubits()
+    one = cpu_baseline(circuit, n, seconds)
+    allc = cpu_baseline_all_cores(circuit, n, seconds / 2)
+    oracle.set_threads(HOST_THREADS)
+    t0 = time.perf_counter()
+    want, _ = oracle.sv_run(circuit.ops(), n)
+    full_s = time.perf_counter() - t0
+    oracle.set_threads(1)
+    got = ev.StateVectorBackend.run(circuit).state().amplitudes()
+    return {"oracle_1_thread": one, "restatement_all_cores": allc,
+            "gpu_over_oracle_1_thread": gpu_gates_per_s / one["value"],
+            "max_abs_vs_oracle": float(np.max(np.abs(got - want))),
+            "oracle_full_run_s": full_s,
+            "parity": f"the fused GPU run of all {len(circuit)} gates against the oracle's full run "
+                      f"({HOST_THREADS} host threads)"}
+
+
 def oracle_fidelity(circuits, idxs, state_of) -> dict:
     """Per-circuit fidelity and max-abs of complex64 states against the FP64 oracle (all host cores)."""
     import numpy as np
@@ -579,6 +605,10 @@ def main() -> int:
         extra["c0"] = {k: r[k] for k in ("workload", "gates", "gates_per_s", "ms_per_step", "device_ms_per_step",
                                          "passes_per_step")}
         log(f"c0: {r['gates_per_s']:.0f} gates/s")
+        if not args.no_cpu:
+            extra["c0"]["cpu"] = c0_cpu_comparison(c0, r["gates_per_s"], min(args.cpu_seconds, 10.0))
+            log(f"c0 CPU: oracle 1 thread {extra['c0']['cpu']['oracle_1_thread']['value']:.1f} gates/s, "
+                f"max-abs vs oracle {extra['c0']['cpu']['max_abs_vs_oracle']:.1e}")
     if wl in ("all", "c2"):
         extra["c2"] = bench_c2(args)
         log(f"c2: {extra['c2']['gates_per_s']:.0f} gates/s")
