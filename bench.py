@@ -297,7 +297,34 @@ def bench_c2(args) -> dict:
         "trace": tr,
         "classical": "".join("1" if b else "0" for b in classical),
         "roofline": roofline(agg, 1, "c2", 26, "c128") if agg else None,
+        "vs_oracle": None if args.no_cpu else c2_oracle_parity(c, noise),
     }
+
+
+def c2_oracle_parity(c, noise) -> dict:
+    """The noisy evolution before the final measurements (which collapse rho to a basis projector)
+    against the oracle's superoperator restatement on all host cores: the same gates and channels as
+    passes over the 26-bit vector, pinned to the oracle's literal loops at n <= 8 by the tests."""
+    import numpy as np
+
+    import everett_amd as ev
+    import oracle
+
+    n, seed = 13, 13
+    body = ev.workloads.dm_random(n, 30, ev.Rng.seed_from_u64(n))
+    ks = np.array([k.m for k in noise.after_1q.ops])
+    oracle.set_threads(HOST_THREADS)
+    t0 = time.perf_counter()
+    try:
+        want, _ = oracle.dm_run(body.ops(), n, {"custom_1q": ks, "custom_2q": ks, "readout_error": 0.01}, seed,
+                                superop=True)
+    finally:
+        oracle.set_threads(1)
+    oracle_s = time.perf_counter() - t0
+    got = ev.DensityMatrixBackend.run_with_noise_seeded(body, noise, seed).density_matrix().matrix().reshape(-1)
+    return {"max_abs": float(np.max(np.abs(got - want))), "oracle_s": oracle_s,
+            "reference": f"oracle superoperator restatement (oracle/refcpu.c, OpenMP on {HOST_THREADS} host threads)",
+            "state": "rho after the 30 noisy layers, before the 13 final measurements"}
 
 
 # ----------------------------------------------------------------------- c3
