@@ -702,10 +702,11 @@ __device__ __forceinline__ void one_tile(const TileRef<R> &tr, void *lds)
 }
 
 // Register budget: 168 VGPRs (3 waves per SIMD; the two c128 register
-// arrays alone take 128).  With the split exchange a tile of 2^t amplitudes
-// takes 2^t reals of LDS, so registers, not LDS, bound the resident waves.
-// (The complex64 kernel takes 77 VGPRs, 6 waves; with matrix-core gates,
-// whose row operands and accumulators live in VGPRs too, 116, 4 waves.)
+// arrays alone take 128; the c128 kernel takes 152, 160 with the sparse
+// 4x4 arm).  With the split exchange a tile of 2^t amplitudes takes 2^t
+// reals of LDS, so registers, not LDS, bound the resident waves.  (The
+// complex64 kernel takes 77 VGPRs, 6 waves; with matrix-core gates, whose
+// row operands and accumulators live in VGPRs too, 116, 4 waves.)
 template <class R> constexpr int kWavesPerSimd = 3;
 
 // XCD-aware tile order: dispatch places workgroup b on XCD b mod 8, so XCD x
