@@ -446,8 +446,8 @@ __device__ __forceinline__ void run_op(const OpView<V> &op, const Regs<V> &in, R
         break;
     }
     case (X ? uint32_t(kU2X) : 0x100u): {  // no such arm (kind < 0x100) unless X
-        // the six coefficients go by value: a local array passed by address
-        // shares a stack slot with the dense arm's row and keeps both in scratch
+        // the six coefficients go by value: passed as a local array they
+        // left this kernel with 464 bytes of scratch per lane
         using F = decltype(V::x);
         const cptr<F> f = (cptr<F>)(coef + op.coef);
         const auto e = [&](int k) { return mk<V>(f[2 * k], f[2 * k + 1]); };
