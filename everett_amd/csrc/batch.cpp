@@ -48,6 +48,9 @@ void batch_free(ev_batch *b)
         for (void *d : b->descs) (void)hipFreeAsync(d, b->stream);  // stream-ordered allocations
         (void)hipStreamSynchronize(b->stream);
     }
+    for (hipGraphExec_t x : b->gexecs) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t x : b->graphs) (void)hipGraphDestroy(x);
+    for (void *h : b->staging) (void)hipHostFree(h);
     if (b->arena) (void)hipFree(b->arena);
     if (b->d_meta) (void)hipFree(b->d_meta);
     if (b->d_part) (void)hipFree(b->d_part);
@@ -327,28 +330,36 @@ ev_batch *batch_run(const Circuit *const *cs, size_t count, const ev_run_options
             b->stats.rounds += max_passes;
             ++b->stats.groups;
         }
+        // ---- f64 renormalisation of every state (c64 drift over deep circuits),
+        // queued behind the last group: f64 norms, 1/sqrt on the device, scale
+        hipEvent_t rn[2] = {nullptr, nullptr};
+        if (opt.renormalize) {
+            EV_HIP(hipEventCreate(&rn[0]));
+            events.push_back(rn[0]);  // destroyed with the group events
+            EV_HIP(hipEventCreate(&rn[1]));
+            events.push_back(rn[1]);
+            double *res = b->d_part + 5 * nchunks, *d_scale = res + 5 * count;
+            EV_HIP(hipEventRecord(rn[0], b->stream));
+            batchk::norms(b->arena, c64, amps, b->d_meta + count + 1, int(count), b->d_part, res, b->stream);
+            batchk::inv_sqrt(res, int(count), d_scale, b->stream);
+            batchk::scale(b->arena, c64, amps, b->d_meta + count + 1, int(count), d_scale, b->stream);
+            EV_HIP(hipEventRecord(rn[1], b->stream));
+        }
         EV_HIP(hipStreamSynchronize(b->stream));
         for (size_t k = 0; k + 1 < events.size(); k += 2) {
             float ms = 0;
             EV_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
-            b->stats.device_ms += ms;
+            (opt.renormalize && k + 2 == events.size() ? b->stats.renorm_ms : b->stats.device_ms) += ms;
         }
+        // the staging buffers and graphs stay with the batch (batch_free)
+        b->staging.swap(pinned);
+        b->graphs.swap(graphs);
+        b->gexecs.swap(gexecs);
         cleanup();
         b->stats.circuits = count;
         for (auto g : b->gates) b->stats.logical_gates += g;
 
-        // ---- f64 renormalisation of every state (c64 drift over deep circuits)
-        if (opt.renormalize) {
-            const double t_rn = now_ms();
-            std::vector<double> nrm(count);
-            batch_norms(b, nrm.data());
-            for (auto &x : nrm) x = x > 0 ? 1.0 / std::sqrt(x) : 0.0;
-            double *d_scale = b->d_part + 5 * nchunks + 5 * count;
-            EV_HIP(hipMemcpyAsync(d_scale, nrm.data(), count * sizeof(double), hipMemcpyHostToDevice, b->stream));
-            batchk::scale(b->arena, c64, amps, b->d_meta + count + 1, int(count), d_scale, b->stream);
-            EV_HIP(hipStreamSynchronize(b->stream));
-            b->stats.renorm_ms = now_ms() - t_rn;
-        }
+
     } catch (...) {
         cleanup();
         batch_free(b);

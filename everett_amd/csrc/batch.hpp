@@ -18,6 +18,12 @@ struct ev_batch {
     uint64_t total_amps = 0;
     void *arena = nullptr;          // all states, back to back
     std::vector<void *> descs;      // per group: pass descriptors and round tables
+    // per group: the pinned staging of its descriptors and its graph, released
+    // with the batch (hipHostFree waits for the device, so freeing them inside
+    // the run would put that wait and the frees on the run's wall time)
+    std::vector<void *> staging;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> gexecs;
     uint64_t *d_meta = nullptr;     // aoff[] then chunk_begin[] on the device
     double *d_part = nullptr;       // reduction scratch
     double *h_out = nullptr;        // pinned results

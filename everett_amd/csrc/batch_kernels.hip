@@ -159,7 +159,20 @@ __global__ void k_scale_states(Amp<R> *a, uint64_t nchunks, const uint64_t *chun
     }
 }
 
+// One thread per state: the renormalisation factor of its f64 norm (IEEE
+// square root and division, as on the host).
+__global__ void k_inv_sqrt(const double *nrm, int count, double *scale)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) scale[i] = nrm[i] > 0 ? 1.0 / sqrt(nrm[i]) : 0.0;
+}
+
 }  // namespace
+
+void inv_sqrt(const double *d_nrm, int count, double *d_scale, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inv_sqrt, dim3((count + 255) / 256), dim3(256), 0, st, d_nrm, count, d_scale);
+}
 
 void init(void *arena, bool c64, uint64_t total_amps, const uint64_t *d_aoff, int count, hipStream_t st)
 {

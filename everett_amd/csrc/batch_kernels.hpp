@@ -20,6 +20,8 @@ void norms(const void *arena, bool c64, uint64_t total_amps, const uint64_t *d_c
 // state, against a c128 arena with the same layout; part holds 5 doubles per chunk.
 void compare(const void *arena, bool c64, const void *ref_arena, uint64_t total_amps, const uint64_t *d_chunk_begin,
              int count, double *part, double *out, hipStream_t st);
+// d_scale[i] = 1 / sqrt(d_nrm[i]) (0 for a zero norm), correctly rounded.
+void inv_sqrt(const double *d_nrm, int count, double *d_scale, hipStream_t st);
 // Multiplies state i by d_scale[i].
 void scale(void *arena, bool c64, uint64_t total_amps, const uint64_t *d_chunk_begin, int count, const double *d_scale,
            hipStream_t st);

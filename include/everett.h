@@ -287,7 +287,7 @@ int ev_dm_run(const ev_circuit *c, const ev_noise_model *noise, uint64_t seed, c
 typedef struct ev_batch ev_batch;
 typedef struct {
     uint64_t circuits, logical_gates, rounds, passes, tiles;
-    double plan_ms, upload_ms, device_ms, renorm_ms;
+    double plan_ms, upload_ms, device_ms, renorm_ms; /* renorm_ms: device time of the f64 renormalisation */
     uint32_t graph;  /* 1 if the groups' rounds ran as captured HIP graphs (one per group) */
     uint32_t groups; /* pipelined groups: the host plans group g+1 while the device runs group g */
     double pass_bytes; /* algorithmic HBM bytes of all fused passes: per pass, every amplitude of its
